@@ -1,0 +1,1331 @@
+// C ABI (include/fqsb.h) of the MI355X implementation of the
+// FrictionQPotSpringBlock dynamics hot path.
+//
+// A `Group` holds R systems of one description in contiguous device arrays
+// ([R, N] per field, [R, N, nchunk] for the landscape). A single system
+// (fqsb_system) is a group of one; a batch (fqsb_batch) is a group of R
+// replicas. The host code is the control flow of the reference's member
+// functions (detail.h:1141-2050): it launches the kernels of fq_resident.hpp
+// and fq_service.hpp and keeps a host mirror of the per-system scalars
+// (RunState) that it pushes before and pulls after each launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fqsb.h"
+#include "fq_launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        throw Error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+    }
+}
+
+#define HIPC(x) hip_check((x), #x)
+
+template <class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return 0;
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+template <class T>
+T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    HIPC(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+    HIPC(hipMemset(p, 0, sizeof(T) * (n ? n : 1)));
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+}
+
+// steps per resident launch of a minimise-type run (bounds one kernel's duration)
+constexpr uint64_t MIN_BLOCK = 1u << 18;
+// steps per resident launch of timeSteps / flowSteps
+constexpr uint64_t STEP_BLOCK = 1u << 16;
+
+struct Group {
+    fqsb_desc d{};
+    int64_t R = 0, N = 0, rows = 0, cols = 0, nchunk = 0;
+    int pot = 0, inter = 0, ext = 0, minim = 0;
+    double *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *f_pot = nullptr, *f_int = nullptr,
+           *f_frame = nullptr, *f_damp = nullptr, *f_th = nullptr, *pref = nullptr, *y = nullptr;
+    int64_t *start = nullptr, *i_ref = nullptr, *next = nullptr, *dinc = nullptr;
+    int32_t* jloc = nullptr;
+    uint64_t *st_begin = nullptr, *st_end = nullptr;
+    fq::RunState* rs = nullptr;
+    fq::SysView* views = nullptr;
+    std::vector<fq::SysView> hviews;
+    std::vector<fq::RunState> hrs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int ppt = 0, maxt = 0, threads = 0;
+    bool user_launch = false;
+    size_t lds_limit = 65536;
+
+    Group() = default;
+    Group(const Group&) = delete;
+    Group& operator=(const Group&) = delete;
+
+    ~Group()
+    {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+        }
+        dfree(u);
+        dfree(v);
+        dfree(a);
+        dfree(f);
+        dfree(f_pot);
+        dfree(f_int);
+        dfree(f_frame);
+        dfree(f_damp);
+        dfree(f_th);
+        dfree(pref);
+        dfree(y);
+        dfree(start);
+        dfree(i_ref);
+        dfree(next);
+        dfree(dinc);
+        dfree(jloc);
+        dfree(st_begin);
+        dfree(st_end);
+        dfree(rs);
+        dfree(views);
+        if (ev0) {
+            (void)hipEventDestroy(ev0);
+        }
+        if (ev1) {
+            (void)hipEventDestroy(ev1);
+        }
+        if (stream) {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void push()
+    {
+        HIPC(hipMemcpyAsync(rs, hrs.data(), sizeof(fq::RunState) * R, hipMemcpyHostToDevice, stream));
+    }
+
+    void pull()
+    {
+        HIPC(hipMemcpyAsync(hrs.data(), rs, sizeof(fq::RunState) * R, hipMemcpyDeviceToHost, stream));
+        HIPC(hipStreamSynchronize(stream));
+    }
+
+    template <class T>
+    void d2h(T* host, const T* dev, size_t n)
+    {
+        HIPC(hipMemcpyAsync(host, dev, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
+        HIPC(hipStreamSynchronize(stream));
+    }
+
+    template <class T>
+    void h2d(T* dev, const T* host, size_t n)
+    {
+        HIPC(hipMemcpyAsync(dev, host, sizeof(T) * n, hipMemcpyHostToDevice, stream));
+        HIPC(hipStreamSynchronize(stream));
+    }
+
+    double* field(int fld) const
+    {
+        switch (fld) {
+            case FQSB_U:
+                return u;
+            case FQSB_V:
+                return v;
+            case FQSB_A:
+                return a;
+            case FQSB_F:
+                return f;
+            case FQSB_F_POTENTIAL:
+                return f_pot;
+            case FQSB_F_INTERACTIONS:
+                return f_int;
+            case FQSB_F_FRAME:
+                return f_frame;
+            case FQSB_F_DAMPING:
+                return f_damp;
+            case FQSB_F_THERMAL:
+                if (!ext) {
+                    throw Error("system has no external forcing");
+                }
+                return f_th;
+            default:
+                throw Error("unknown field");
+        }
+    }
+
+    void forces(int32_t flags)
+    {
+        push();
+        if (!fq::launch_forces(pot, inter, ext, views, (int)R, flags, stream)) {
+            throw Error("no force kernel for this potential / interaction / external combination");
+        }
+        HIPC(hipGetLastError());
+        pull();
+    }
+
+    void updated_u() // detail.h:1380-1386
+    {
+        forces(fq::FF_POT | fq::FF_INT | fq::FF_FRAME | fq::FF_TOTAL);
+    }
+
+    void updated_v() // detail.h:1391-1395
+    {
+        forces(fq::FF_DAMP | fq::FF_TOTAL);
+    }
+
+    void updated_inc() // detail.h:1369-1375
+    {
+        forces(ext ? (fq::FF_THERMAL | fq::FF_TOTAL) : fq::FF_TOTAL);
+    }
+
+    void refresh() // detail.h:1310-1315
+    {
+        forces(fq::FF_POT | fq::FF_INT | fq::FF_FRAME | fq::FF_DAMP | fq::FF_TOTAL | (ext ? fq::FF_THERMAL : 0));
+    }
+
+    // Pick the resident variant: fewest particles per thread that fits, then
+    // the smallest compiled thread bound (most registers per thread).
+    void choose_launch(int want_ppt, int want_threads)
+    {
+        int vp[64], vm[64];
+        int nv = std::min(fq::resident_variants(pot, inter, ext, vp, vm, 64), 64);
+        if (nv <= 0) {
+            throw Error("no resident kernel for this potential / interaction / external combination");
+        }
+        int best = -1, best_thr = 0;
+        for (int i = 0; i < nv; ++i) {
+            const int p = vp[i], mt = vm[i];
+            if (want_ppt > 0 && p != want_ppt) {
+                continue;
+            }
+            int64_t need = (N + p - 1) / p;
+            int thr = want_threads > 0 ? want_threads : (int)(((need + 63) / 64) * 64);
+            if (thr % 64 != 0 || thr > mt || (int64_t)thr * p < N || thr > 1024) {
+                continue;
+            }
+            if (fq::resident_lds(inter, N, thr) > lds_limit) {
+                continue;
+            }
+            if (best < 0 || p < vp[best] || (p == vp[best] && mt < vm[best])) {
+                best = i;
+                best_thr = thr;
+            }
+        }
+        if (best < 0) {
+            throw Error("system size " + std::to_string(N) +
+                        " is outside the capacity of the resident kernels for this interaction");
+        }
+        ppt = vp[best];
+        maxt = vm[best];
+        threads = best_thr;
+    }
+
+    void launch_resident(const fq::RunArgs& ra)
+    {
+        if (minim == FQSB_MIN_OVERDAMPED) {
+            throw Error("dynamics are not available for no-passing systems");
+        }
+        size_t lds = fq::resident_lds(inter, N, threads);
+        if (!fq::launch_resident(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream)) {
+            throw Error("no resident kernel variant for this launch shape");
+        }
+        HIPC(hipGetLastError());
+    }
+
+    void reset_run(int mode)
+    {
+        for (auto& r : hrs) {
+            r.status = fq::ST_RUNNING;
+            r.steps = 0;
+            r.below = 0;
+            r.below2 = 0;
+            r.dec = 0;
+            r.prev = std::numeric_limits<double>::infinity();
+            if (mode == fq::MODE_TRUNCATE || mode == fq::MODE_MINIMISE) {
+                r.act_init = 1;
+                r.s_n = 0;
+            }
+        }
+    }
+
+    bool any_running() const
+    {
+        for (const auto& r : hrs) {
+            if (r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) {
+                return true;
+            }
+        }
+        return false;
+    }
+
+    // Run `total` steps (or until every system stopped) in launches of `block`.
+    // Returns the accumulated time between events around the launches (ms).
+    float run(fq::RunArgs ra, uint64_t total, uint64_t block, bool check_each, bool timed)
+    {
+        reset_run(ra.mode);
+        push();
+        if (timed) {
+            HIPC(hipEventRecord(ev0, stream));
+        }
+        uint64_t done = 0;
+        while (done < total) {
+            const uint64_t nb = std::min(block, total - done);
+            ra.max_steps = (int64_t)nb;
+            launch_resident(ra);
+            done += nb;
+            if (check_each && done < total) {
+                pull();
+                if (!any_running()) {
+                    break;
+                }
+                bool nan = false;
+                for (const auto& r : hrs) {
+                    nan |= r.status == fq::ST_NAN;
+                }
+                if (nan) {
+                    break;
+                }
+            }
+        }
+        float ms = 0.0f;
+        if (timed) {
+            HIPC(hipEventRecord(ev1, stream));
+            HIPC(hipEventSynchronize(ev1));
+            HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        }
+        pull();
+        return ms;
+    }
+
+    float steps(uint64_t n, bool flow, double v_frame, uint64_t block, bool check_each, bool timed)
+    {
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_STEPS;
+        ra.flow = flow ? 1 : 0;
+        ra.v_frame = v_frame;
+        float ms = run(ra, n, block ? block : STEP_BLOCK, check_each, timed);
+        for (const auto& r : hrs) {
+            if (r.status == fq::ST_NAN) {
+                throw Error("NaN entries found");
+            }
+        }
+        return ms;
+    }
+
+    void set_iref_current()
+    {
+        HIPC(fq::launch_set_iref(views, (int)R, N, stream));
+    }
+
+    // quasistatic helpers -----------------------------------------------------
+
+    double max_uniform(int64_t r, int direction, std::vector<double>* all = nullptr)
+    {
+        if (pot == FQSB_POT_SMOOTH) {
+            throw Error("Operation not possible.");
+        }
+        push();
+        if (!fq::launch_max_uniform(pot, views, (int)R, direction, stream)) {
+            throw Error("maxUniformDisplacement not available");
+        }
+        HIPC(hipGetLastError());
+        pull();
+        if (all) {
+            all->resize(R);
+            for (int64_t i = 0; i < R; ++i) {
+                (*all)[i] = hrs[i].red_flag ? 0.0 : hrs[i].red0;
+            }
+        }
+        return hrs[r].red_flag ? 0.0 : hrs[r].red0;
+    }
+
+    // advanceUniformly detail.h:2027-2050 (input_is_frame = false), per system
+    void advance_uniformly(const std::vector<double>& du, std::vector<double>* du_frame)
+    {
+        if (du_frame) {
+            du_frame->assign(R, 0.0);
+        }
+        for (int64_t r = 0; r < R; ++r) {
+            const double dp = du[r];
+            const double dfr = dp * (d.k_frame + d.mu) / d.k_frame;
+            hrs[r].du_add = dp;
+            hrs[r].u_frame += dfr;
+            if (du_frame) {
+                (*du_frame)[r] = dfr;
+            }
+        }
+        forces(fq::FF_ADD_U | fq::FF_POT | fq::FF_INT | fq::FF_FRAME | fq::FF_TOTAL);
+        for (auto& r : hrs) {
+            r.du_add = 0.0;
+        }
+    }
+
+    // eventDrivenStep detail.h:1933-1960 for every system
+    std::vector<double> event_driven_step(double eps, bool kick, int direction)
+    {
+        std::vector<double> du(R, 0.0), ret(R, 0.0);
+        if (kick) {
+            for (int64_t r = 0; r < R; ++r) {
+                du[r] = direction > 0 ? eps : -eps;
+            }
+            advance_uniformly(du, &ret);
+            return ret;
+        }
+        std::vector<double> m;
+        max_uniform(0, direction, &m);
+        bool any = false;
+        for (int64_t r = 0; r < R; ++r) {
+            if (m[r] < 0.5 * eps) {
+                du[r] = 0.0;
+            }
+            else {
+                du[r] = direction > 0 ? m[r] - 0.5 * eps : 0.5 * eps - m[r];
+                any = true;
+            }
+        }
+        if (any) {
+            advance_uniformly(du, &ret);
+            for (int64_t r = 0; r < R; ++r) {
+                if (m[r] < 0.5 * eps) {
+                    ret[r] = 0.0;
+                }
+            }
+        }
+        return ret;
+    }
+};
+
+uint64_t default_stream_state(uint64_t seed, uint64_t& inc)
+{
+    uint64_t st;
+    fq::pcg_seed(seed, fq::PCG_DEFAULT_STREAM, st, inc);
+    return st;
+}
+
+Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride)
+{
+    if (!dsc) {
+        throw Error("null description");
+    }
+    const fqsb_desc& d = *dsc;
+    if (d.rank != 1 && d.rank != 2) {
+        throw Error("rank must be 1 or 2");
+    }
+    if (d.distribution < 0 || d.distribution > FQSB_DIST_NORMAL) {
+        throw Error("Unknown distribution: " + std::to_string(d.distribution));
+    }
+    if (d.distribution == FQSB_DIST_GAMMA || d.distribution == FQSB_DIST_NORMAL) {
+        throw Error("Distribution not supported for the yield-position landscape");
+    }
+    if (d.nparameters < 0 || d.nparameters > 4) {
+        throw Error("Too many parameters for distribution");
+    }
+    const int maxpar = (d.distribution == FQSB_DIST_POWER || d.distribution == FQSB_DIST_PARETO ||
+                        d.distribution == FQSB_DIST_WEIBULL)
+                           ? 3
+                           : 2;
+    if (d.nparameters > maxpar) {
+        throw Error("Too many parameters for distribution");
+    }
+    if (d.nchunk < 8) {
+        throw Error("nchunk must be at least 8");
+    }
+    if (replicas < 1) {
+        throw Error("at least one replica is needed");
+    }
+    const int64_t rows = (int64_t)d.rows, cols = d.rank == 1 ? 1 : (int64_t)d.cols;
+    const int64_t N = rows * cols;
+    if (N < 1) {
+        throw Error("empty system");
+    }
+    const int inter = d.interactions;
+    if ((inter == FQSB_INT_LAPLACE1D || inter == FQSB_INT_QUARTIC1D || inter == FQSB_INT_QUARTICGRADIENT1D) &&
+        N < 3) {
+        throw Error("a chain needs at least 3 particles");
+    }
+    if ((inter == FQSB_INT_LAPLACE2D || inter == FQSB_INT_QUARTICGRADIENT2D) && d.rank != 2) {
+        throw Error("2d interactions need rank 2");
+    }
+    if (d.minimisation == FQSB_MIN_OVERDAMPED && (inter != FQSB_INT_LAPLACE1D || d.potential != FQSB_POT_CUSPY)) {
+        throw Error("no-passing minimisation needs Cuspy + Laplace1d");
+    }
+    if (d.external == FQSB_EXT_RANDOMFORCING && (!d.dinc_init || !d.dinc)) {
+        throw Error("random forcing needs dinc_init and dinc");
+    }
+
+    std::unique_ptr<Group> g(new Group());
+    g->d = d;
+    g->d.dinc_init = nullptr;
+    g->d.dinc = nullptr;
+    g->R = (int64_t)replicas;
+    g->N = N;
+    g->rows = rows;
+    g->cols = cols;
+    g->nchunk = (int64_t)d.nchunk;
+    g->pot = d.potential;
+    g->inter = inter;
+    g->ext = d.external;
+    g->minim = d.minimisation;
+
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, dev));
+    g->lds_limit = prop.sharedMemPerBlock - 2048;
+    HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreate(&g->ev0));
+    HIPC(hipEventCreate(&g->ev1));
+
+    const int64_t R = g->R;
+    const size_t RN = (size_t)(R * N);
+    g->u = dalloc<double>(RN);
+    g->v = dalloc<double>(RN);
+    g->a = dalloc<double>(RN);
+    g->f = dalloc<double>(RN);
+    g->f_pot = dalloc<double>(RN);
+    g->f_int = dalloc<double>(RN);
+    g->f_frame = dalloc<double>(RN);
+    g->f_damp = dalloc<double>(RN);
+    g->y = dalloc<double>(RN * (size_t)g->nchunk);
+    g->start = dalloc<int64_t>(RN);
+    g->i_ref = dalloc<int64_t>(RN);
+    g->jloc = dalloc<int32_t>(RN);
+    g->st_begin = dalloc<uint64_t>(RN);
+    g->st_end = dalloc<uint64_t>(RN);
+    g->rs = dalloc<fq::RunState>((size_t)R);
+    g->views = dalloc<fq::SysView>((size_t)R);
+    if (g->ext) {
+        g->f_th = dalloc<double>(RN);
+        g->next = dalloc<int64_t>(RN);
+        g->dinc = dalloc<int64_t>(RN);
+        for (int64_t r = 0; r < R; ++r) {
+            g->h2d(g->next + r * N, d.dinc_init, (size_t)N);
+            g->h2d(g->dinc + r * N, d.dinc, (size_t)N);
+        }
+    }
+    int64_t lr_m = 0;
+    if (inter == FQSB_INT_LONGRANGE1D) { // LongRange1d ctor detail.h:829-844
+        lr_m = (N - N % 2) / 2;
+        std::vector<double> pr((size_t)N, 0.0);
+        for (int64_t dd = 1; dd < N; ++dd) {
+            pr[(size_t)dd] = d.k_interactions / std::pow((double)dd, d.alpha + 1.0);
+        }
+        g->pref = dalloc<double>((size_t)N);
+        g->h2d(g->pref, pr.data(), (size_t)N);
+    }
+
+    // distribution parameters with the reference's defaults (see fq_common.hpp)
+    double p[3] = {1.0, 0.0, 0.0};
+    if (maxpar == 3) {
+        p[1] = 1.0;
+    }
+    for (int i = 0; i < d.nparameters; ++i) {
+        p[i] = d.parameters[i];
+    }
+    // alignment (own rule standing in for prrng::alignment(2, 30, 6, false))
+    const int64_t n = g->nchunk;
+    int64_t bb = d.align_buffer < 0 ? 0 : d.align_buffer;
+    bb = std::min<int64_t>(bb, (n - 2) / 4);
+    int64_t mm = std::max<int64_t>(d.align_margin, bb);
+    mm = std::min<int64_t>(mm, n - 2 - bb);
+
+    g->hviews.resize((size_t)R);
+    g->hrs.assign((size_t)R, fq::RunState{});
+    for (int64_t r = 0; r < R; ++r) {
+        fq::SysView& S = g->hviews[(size_t)r];
+        std::memset(&S, 0, sizeof(S));
+        S.N = N;
+        S.rows = rows;
+        S.cols = cols;
+        S.pot = g->pot;
+        S.inter = inter;
+        S.ext = g->ext;
+        S.m = d.m;
+        S.inv_m = 1.0 / d.m;
+        S.eta = d.eta;
+        S.mu = d.mu;
+        S.kappa = d.kappa;
+        S.k_frame = d.k_frame;
+        S.dt = d.dt;
+        S.k = d.k_interactions;
+        S.a1 = d.a1;
+        S.a2 = d.a2;
+        S.k2 = d.k2;
+        S.k4 = d.k4;
+        S.k4_4 = 0.25 * d.k4;
+        S.alpha = d.alpha;
+        S.lr_m = lr_m;
+        S.pref = g->pref;
+        const size_t o = (size_t)(r * N);
+        S.u = g->u + o;
+        S.v = g->v + o;
+        S.a = g->a + o;
+        S.f = g->f + o;
+        S.f_pot = g->f_pot + o;
+        S.f_int = g->f_int + o;
+        S.f_frame = g->f_frame + o;
+        S.f_damp = g->f_damp + o;
+        S.f_th = g->f_th ? g->f_th + o : nullptr;
+        S.i_ref = g->i_ref + o;
+        S.next = g->next ? g->next + o : nullptr;
+        S.dinc = g->dinc ? g->dinc + o : nullptr;
+        S.mean = d.mean;
+        S.stddev = d.stddev;
+        S.seed = d.seed + (uint64_t)r * seed_stride;
+        S.offset = d.offset;
+        S.ls.y = g->y + o * (size_t)n;
+        S.ls.start = g->start + o;
+        S.ls.jloc = g->jloc + o;
+        S.ls.st_begin = g->st_begin + o;
+        S.ls.st_end = g->st_end + o;
+        S.ls.nchunk = n;
+        S.ls.buffer = bb;
+        S.ls.margin = mm;
+        S.ls.inc = 1u; // (initseq 0 << 1) | 1
+        S.ls.dist = d.distribution;
+        S.ls.p0 = p[0];
+        S.ls.p1 = p[1];
+        S.ls.p2 = p[2];
+        S.rs = g->rs + r;
+        fq::RunState& rs = g->hrs[(size_t)r];
+        rs.inc = 0;
+        rs.u_frame = 0.0;
+        rs.prev = std::numeric_limits<double>::infinity();
+        rs.status = fq::ST_RUNNING;
+        uint64_t rinc = 0;
+        rs.rng_state = default_stream_state(d.seed_forcing, rinc);
+        rs.rng_inc = rinc;
+        rs.min_next = std::numeric_limits<int64_t>::max();
+    }
+    g->h2d(g->views, g->hviews.data(), (size_t)R);
+    g->push();
+    HIPC(fq::launch_landscape_init(g->views, (int)R, N, g->stream));
+    g->refresh();
+    if (g->minim != FQSB_MIN_OVERDAMPED) {
+        g->choose_launch(0, 0);
+    }
+    return g.release();
+}
+
+int64_t checked_p(const Group& g, uint64_t p)
+{
+    if ((int64_t)p >= g.N) {
+        throw Error("particle index out of range");
+    }
+    return (int64_t)p;
+}
+
+} // namespace
+
+struct fqsb_system {
+    std::unique_ptr<Group> g;
+};
+
+struct fqsb_batch {
+    std::unique_ptr<Group> g;
+};
+
+extern "C" {
+
+const char* fqsb_last_error(void)
+{
+    return g_err.c_str();
+}
+
+const char* fqsb_version(void)
+{
+    return "frictionqpotspringblock_amd 0.1.0 (gfx950)";
+}
+
+int fqsb_create(const fqsb_desc* desc, fqsb_system** out)
+{
+    return guard([&] {
+        *out = nullptr;
+        std::unique_ptr<fqsb_system> s(new fqsb_system());
+        s->g.reset(build_group(desc, 1, 0));
+        *out = s.release();
+    });
+}
+
+int fqsb_destroy(fqsb_system* s)
+{
+    delete s;
+    return 0;
+}
+
+uint64_t fqsb_size(const fqsb_system* s)
+{
+    return (uint64_t)s->g->N;
+}
+
+int fqsb_get(fqsb_system* s, int field, double* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.d2h(out, g.field(field), (size_t)g.N);
+    });
+}
+
+int fqsb_set(fqsb_system* s, int field, const double* in)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        switch (field) {
+            case FQSB_U:
+                g.h2d(g.u, in, (size_t)g.N);
+                g.updated_u();
+                break;
+            case FQSB_V:
+                g.h2d(g.v, in, (size_t)g.N);
+                g.updated_v();
+                break;
+            case FQSB_A:
+                g.h2d(g.a, in, (size_t)g.N);
+                break;
+            case FQSB_F_THERMAL:
+                g.h2d(g.field(FQSB_F_THERMAL), in, (size_t)g.N);
+                break;
+            default:
+                throw Error("field is read-only");
+        }
+    });
+}
+
+int fqsb_get_inc(const fqsb_system* s, int64_t* out)
+{
+    *out = s->g->hrs[0].inc;
+    return 0;
+}
+
+int fqsb_set_inc(fqsb_system* s, int64_t inc)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.hrs[0].inc = inc;
+        g.hrs[0].qs_first = inc;
+        g.hrs[0].qs_last = inc;
+        g.updated_inc();
+    });
+}
+
+int fqsb_set_t(fqsb_system* s, double t)
+{
+    Group& g = *s->g;
+    g.hrs[0].inc = (int64_t)std::round(t / g.d.dt);
+    return 0;
+}
+
+int fqsb_get_u_frame(const fqsb_system* s, double* out)
+{
+    *out = s->g->hrs[0].u_frame;
+    return 0;
+}
+
+int fqsb_set_u_frame(fqsb_system* s, double u_frame)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.hrs[0].u_frame = u_frame;
+        g.forces(fq::FF_FRAME | fq::FF_TOTAL);
+    });
+}
+
+int fqsb_quasistatic_activity(const fqsb_system* s, int64_t* first, int64_t* last)
+{
+    *first = s->g->hrs[0].qs_first;
+    *last = s->g->hrs[0].qs_last;
+    return 0;
+}
+
+static double norm_sq(const std::vector<double>& x)
+{
+    double r = 0.0;
+    for (double v : x) {
+        r += v * v;
+    }
+    return r;
+}
+
+int fqsb_residual(fqsb_system* s, double* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        std::vector<double> f((size_t)g.N), ff((size_t)g.N);
+        g.d2h(f.data(), g.f, f.size());
+        g.d2h(ff.data(), g.f_frame, ff.size());
+        const double r_fres = std::sqrt(norm_sq(f));
+        const double r_fext = std::sqrt(norm_sq(ff));
+        *out = r_fext != 0.0 ? r_fres / r_fext : r_fres;
+    });
+}
+
+int fqsb_temperature(fqsb_system* s, double* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        std::vector<double> v((size_t)g.N);
+        g.d2h(v.data(), g.v, v.size());
+        *out = 0.5 * g.d.m * norm_sq(v) / (double)g.N;
+    });
+}
+
+int fqsb_refresh(fqsb_system* s)
+{
+    return guard([&] { s->g->refresh(); });
+}
+
+int fqsb_quench(fqsb_system* s)
+{
+    return guard([&] { s->g->forces(fq::FF_QUENCH | fq::FF_DAMP | fq::FF_TOTAL); });
+}
+
+int fqsb_time_steps(fqsb_system* s, uint64_t n)
+{
+    return guard([&] {
+        if (n > 0) {
+            s->g->steps(n, false, 0.0, 0, true, false);
+        }
+    });
+}
+
+int fqsb_time_step(fqsb_system* s)
+{
+    return fqsb_time_steps(s, 1);
+}
+
+int fqsb_flow_steps(fqsb_system* s, uint64_t n, double v_frame)
+{
+    return guard([&] {
+        if (n > 0) {
+            s->g->steps(n, true, v_frame, 0, true, false);
+        }
+    });
+}
+
+int fqsb_time_steps_until_event(fqsb_system* s, double tol, uint64_t niter_tol, uint64_t max_iter, uint64_t* ret)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.set_iref_current();
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_UNTIL_EVENT;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        const auto& r = g.hrs[0];
+        if (r.status == fq::ST_NAN) {
+            throw Error("NaN entries found");
+        }
+        if (r.status == fq::ST_EVENT) {
+            *ret = (uint64_t)r.steps;
+        }
+        else if (r.status == fq::ST_CONVERGED) {
+            *ret = 0;
+        }
+        else {
+            *ret = max_iter + 1;
+        }
+    });
+}
+
+static void minimise_nopassing(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                               uint64_t* ret)
+{
+    // detail.h:1694-1753 on the device (Jacobi sweeps)
+    g.hrs[0].qs_first = g.hrs[0].inc;
+    g.hrs[0].qs_last = g.hrs[0].inc;
+    g.reset_run(fq::MODE_MINIMISE);
+    g.push();
+    int ppt = 1;
+    while ((g.N + ppt - 1) / ppt > 1024) {
+        ppt *= 2;
+    }
+    if (ppt > 16) {
+        throw Error("system too large for the no-passing kernel");
+    }
+    const int64_t need = (g.N + ppt - 1) / ppt;
+    const int threads = (int)(((need + 63) / 64) * 64);
+    fq::RunArgs ra{};
+    ra.mode = fq::MODE_MINIMISE;
+    ra.tol = tol;
+    ra.tol2 = tol * tol;
+    ra.niter_tol = (int64_t)niter_tol;
+    uint64_t done = 0;
+    while (done < max_iter) {
+        const uint64_t nb = std::min<uint64_t>(MIN_BLOCK, max_iter - done);
+        ra.max_steps = (int64_t)nb;
+        if (!fq::launch_nopassing(ppt, threads, sizeof(double) * 4 * threads, g.views, 1, ra, g.stream)) {
+            throw Error("no-passing kernel not available");
+        }
+        HIPC(hipGetLastError());
+        done += nb;
+        g.pull();
+        if (g.hrs[0].status == fq::ST_CONVERGED) {
+            break;
+        }
+    }
+    if (g.hrs[0].status == fq::ST_CONVERGED) {
+        *ret = 0;
+        return;
+    }
+    if (max_iter_is_error) {
+        throw Error("No convergence found");
+    }
+    *ret = max_iter + 1;
+}
+
+int fqsb_minimise(fqsb_system* s, double tol, uint64_t niter_tol, uint64_t max_iter, int time_activity,
+                  int max_iter_is_error, uint64_t* ret)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        if (g.minim == FQSB_MIN_NONE) {
+            throw Error("Minimisation not implementated");
+        }
+        if (g.minim == FQSB_MIN_OVERDAMPED) {
+            minimise_nopassing(g, tol, niter_tol, max_iter, max_iter_is_error, ret);
+            return;
+        }
+        if (time_activity) {
+            g.set_iref_current();
+        }
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_MINIMISE;
+        ra.time_activity = time_activity ? 1 : 0;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        const auto& r = g.hrs[0];
+        if (r.status == fq::ST_NAN) {
+            throw Error("NaN entries found");
+        }
+        if (r.status == fq::ST_CONVERGED) {
+            *ret = 0;
+            return;
+        }
+        if (max_iter_is_error) {
+            throw Error("No convergence found");
+        }
+        *ret = max_iter + 1;
+    });
+}
+
+int fqsb_minimise_truncate(fqsb_system* s, const int64_t* i_n, uint64_t A_truncate, uint64_t S_truncate, double tol,
+                           uint64_t niter_tol, uint64_t max_iter, int time_activity, int max_iter_is_error,
+                           uint64_t* ret)
+{
+    (void)time_activity;
+    return guard([&] {
+        Group& g = *s->g;
+        if (g.minim != FQSB_MIN_INERTIAL) {
+            throw Error("minimise_truncate is not available for this system");
+        }
+        g.h2d(g.i_ref, i_n, (size_t)g.N);
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_TRUNCATE;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        ra.A_truncate = (int64_t)A_truncate;
+        ra.S_truncate = (int64_t)S_truncate;
+        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        const auto& r = g.hrs[0];
+        if (r.status == fq::ST_NAN) {
+            throw Error("NaN entries found");
+        }
+        if (r.status == fq::ST_CONVERGED) {
+            *ret = 0;
+            return;
+        }
+        if (r.status == fq::ST_TRUNCATED) {
+            *ret = (uint64_t)r.steps;
+            return;
+        }
+        if (max_iter_is_error) {
+            throw Error("No convergence found");
+        }
+        *ret = max_iter + 1;
+    });
+}
+
+int fqsb_max_uniform_displacement(fqsb_system* s, int direction, double* ret)
+{
+    return guard([&] { *ret = s->g->max_uniform(0, direction); });
+}
+
+int fqsb_event_driven_step(fqsb_system* s, double eps, int kick, int direction, double* ret)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        if (g.ext) {
+            throw Error("eventDrivenStep is not available for this system");
+        }
+        *ret = g.event_driven_step(eps, kick != 0, direction)[0];
+    });
+}
+
+int fqsb_trigger(fqsb_system* s, uint64_t p, double eps, int direction)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        const int64_t pp = checked_p(g, p);
+        HIPC(fq::launch_trigger(g.views, pp, eps, direction, g.pot == FQSB_POT_CUSPY ? 1 : 0, g.stream));
+        HIPC(hipStreamSynchronize(g.stream));
+    });
+}
+
+int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plastic)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        const size_t N = (size_t)g.N;
+        std::vector<int64_t> st(N), st2(N);
+        std::vector<int32_t> jl(N), jl2(N);
+        std::vector<double> ff(N);
+        g.d2h(st.data(), g.start, N);
+        g.d2h(jl.data(), g.jloc, N);
+        g.d2h(ff.data(), g.f_frame, N);
+        double mean = 0.0;
+        for (double x : ff) {
+            mean += x;
+        }
+        mean /= (double)N;
+        std::vector<double> du(1, (f_frame - mean) / g.d.mu);
+        g.advance_uniformly(du, nullptr);
+        g.d2h(st2.data(), g.start, N);
+        g.d2h(jl2.data(), g.jloc, N);
+        bool same = true;
+        for (size_t p = 0; p < N; ++p) {
+            same &= st[p] + jl[p] == st2[p] + jl2[p];
+        }
+        if (!allow_plastic && !same) {
+            throw Error("assertion failed (allow_plastic || xt::all(xt::equal(m_chunk->index_at_align(), i_n)))");
+        }
+    });
+}
+
+uint64_t fqsb_chunk_size(const fqsb_system* s)
+{
+    return (uint64_t)s->g->nchunk;
+}
+
+int fqsb_chunk_data(fqsb_system* s, double* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.d2h(out, g.y, (size_t)(g.N * g.nchunk));
+    });
+}
+
+int fqsb_chunk_index_at_align(fqsb_system* s, int64_t* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        std::vector<int32_t> jl((size_t)g.N);
+        g.d2h(out, g.start, (size_t)g.N);
+        g.d2h(jl.data(), g.jloc, (size_t)g.N);
+        for (int64_t p = 0; p < g.N; ++p) {
+            out[p] += jl[(size_t)p];
+        }
+    });
+}
+
+int fqsb_chunk_chunk_index_at_align(fqsb_system* s, int64_t* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        std::vector<int32_t> jl((size_t)g.N);
+        g.d2h(jl.data(), g.jloc, (size_t)g.N);
+        for (int64_t p = 0; p < g.N; ++p) {
+            out[p] = jl[(size_t)p];
+        }
+    });
+}
+
+int fqsb_chunk_start(fqsb_system* s, int64_t* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        g.d2h(out, g.start, (size_t)g.N);
+    });
+}
+
+int fqsb_chunk_state_at(fqsb_system* s, const int64_t* index, uint64_t* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        std::vector<int64_t> st((size_t)g.N);
+        std::vector<uint64_t> sb((size_t)g.N);
+        g.d2h(st.data(), g.start, st.size());
+        g.d2h(sb.data(), g.st_begin, sb.size());
+        for (int64_t p = 0; p < g.N; ++p) {
+            out[p] = fq::pcg_advance(sb[(size_t)p], 1u, index[p] - st[(size_t)p]);
+        }
+    });
+}
+
+int fqsb_chunk_restore(fqsb_system* s, const uint64_t* state, const double* value, const int64_t* index)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        const size_t N = (size_t)g.N;
+        uint64_t* dstate = dalloc<uint64_t>(N);
+        double* dvalue = dalloc<double>(N);
+        int64_t* dindex = dalloc<int64_t>(N);
+        try {
+            g.h2d(dstate, state, N);
+            g.h2d(dvalue, value, N);
+            g.h2d(dindex, index, N);
+            HIPC(fq::launch_restore(g.views, g.N, dstate, dvalue, dindex, g.stream));
+            HIPC(hipStreamSynchronize(g.stream));
+        }
+        catch (...) {
+            dfree(dstate);
+            dfree(dvalue);
+            dfree(dindex);
+            throw;
+        }
+        dfree(dstate);
+        dfree(dvalue);
+        dfree(dindex);
+    });
+}
+
+int fqsb_thermal_state(const fqsb_system* s, uint64_t* out)
+{
+    *out = s->g->hrs[0].rng_state;
+    return 0;
+}
+
+int fqsb_thermal_set_state(fqsb_system* s, uint64_t state)
+{
+    s->g->hrs[0].rng_state = state;
+    return 0;
+}
+
+int fqsb_thermal_next(fqsb_system* s, int64_t* out)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        if (!g.ext) {
+            throw Error("system has no external forcing");
+        }
+        g.d2h(out, g.next, (size_t)g.N);
+    });
+}
+
+int fqsb_thermal_set_next(fqsb_system* s, const int64_t* in)
+{
+    return guard([&] {
+        Group& g = *s->g;
+        if (!g.ext) {
+            throw Error("system has no external forcing");
+        }
+        g.h2d(g.next, in, (size_t)g.N);
+    });
+}
+
+// --- batch -------------------------------------------------------------------
+
+int fqsb_batch_create(const fqsb_desc* desc, uint64_t replicas, uint64_t seed_stride, fqsb_batch** out)
+{
+    return guard([&] {
+        *out = nullptr;
+        std::unique_ptr<fqsb_batch> b(new fqsb_batch());
+        b->g.reset(build_group(desc, replicas, seed_stride));
+        *out = b.release();
+    });
+}
+
+int fqsb_batch_destroy(fqsb_batch* b)
+{
+    delete b;
+    return 0;
+}
+
+uint64_t fqsb_batch_replicas(const fqsb_batch* b)
+{
+    return (uint64_t)b->g->R;
+}
+
+int fqsb_batch_set_launch(fqsb_batch* b, int ppt, int threads)
+{
+    return guard([&] { b->g->choose_launch(ppt, threads); });
+}
+
+int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads)
+{
+    *ppt = b->g->ppt;
+    *threads = b->g->threads;
+    return 0;
+}
+
+int fqsb_batch_flow_steps(fqsb_batch* b, uint64_t n, double v_frame, uint64_t block, float* ms)
+{
+    return guard([&] {
+        float t = b->g->steps(n, true, v_frame, block, false, ms != nullptr);
+        if (ms) {
+            *ms = t;
+        }
+    });
+}
+
+int fqsb_batch_time_steps(fqsb_batch* b, uint64_t n, uint64_t block, float* ms)
+{
+    return guard([&] {
+        float t = b->g->steps(n, false, 0.0, block, false, ms != nullptr);
+        if (ms) {
+            *ms = t;
+        }
+    });
+}
+
+int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                        uint64_t* ret)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        if (g.minim != FQSB_MIN_INERTIAL) {
+            throw Error("batched minimise needs inertial systems");
+        }
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_MINIMISE;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        for (int64_t r = 0; r < g.R; ++r) {
+            const auto& st = g.hrs[(size_t)r];
+            if (st.status == fq::ST_NAN) {
+                throw Error("NaN entries found");
+            }
+            if (st.status == fq::ST_CONVERGED) {
+                ret[r] = 0;
+            }
+            else if (max_iter_is_error) {
+                throw Error("No convergence found");
+            }
+            else {
+                ret[r] = max_iter + 1;
+            }
+        }
+    });
+}
+
+int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direction, double* ret)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        if (g.ext) {
+            throw Error("eventDrivenStep is not available for this system");
+        }
+        auto r = g.event_driven_step(eps, kick != 0, direction);
+        std::copy(r.begin(), r.end(), ret);
+    });
+}
+
+int fqsb_batch_get(fqsb_batch* b, int field, double* out)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        g.d2h(out, g.field(field), (size_t)(g.R * g.N));
+    });
+}
+
+int fqsb_batch_set(fqsb_batch* b, int field, const double* in)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        const size_t n = (size_t)(g.R * g.N);
+        switch (field) {
+            case FQSB_U:
+                g.h2d(g.u, in, n);
+                g.updated_u();
+                break;
+            case FQSB_V:
+                g.h2d(g.v, in, n);
+                g.updated_v();
+                break;
+            case FQSB_A:
+                g.h2d(g.a, in, n);
+                break;
+            default:
+                throw Error("field is read-only");
+        }
+    });
+}
+
+int fqsb_batch_get_u_frame(fqsb_batch* b, double* out)
+{
+    for (int64_t r = 0; r < b->g->R; ++r) {
+        out[r] = b->g->hrs[(size_t)r].u_frame;
+    }
+    return 0;
+}
+
+int fqsb_batch_get_inc(fqsb_batch* b, int64_t* out)
+{
+    for (int64_t r = 0; r < b->g->R; ++r) {
+        out[r] = b->g->hrs[(size_t)r].inc;
+    }
+    return 0;
+}
+
+int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        const size_t n = (size_t)(g.R * g.N);
+        std::vector<int32_t> jl(n);
+        g.d2h(out, g.start, n);
+        g.d2h(jl.data(), g.jloc, n);
+        for (size_t p = 0; p < n; ++p) {
+            out[p] += jl[p];
+        }
+    });
+}
+
+void* fqsb_batch_stream(fqsb_batch* b)
+{
+    return (void*)b->g->stream;
+}
+
+} // extern "C"

@@ -1,0 +1,523 @@
+// Resident Velocity-Verlet kernel: one workgroup owns one system for many
+// steps. Each thread keeps PPT consecutive particles (u, v, a, the two yield
+// positions of the current well, its local well index) in registers for the
+// whole launch; neighbours are exchanged through LDS. It implements
+//   timeStep            detail.h:1539-1570
+//   timeSteps/flowSteps detail.h:1577-1583, 1637-1645   (MODE_STEPS)
+//   timeStepsUntilEvent detail.h:1595-1622              (MODE_UNTIL_EVENT)
+//   minimise (inertial) detail.h:1754-1785              (MODE_MINIMISE)
+//   minimise_truncate   detail.h:1858-1886              (MODE_TRUNCATE)
+//   RandomNormalForcing::force detail.h:930-943         (EXT = 1)
+//
+// One __syncthreads per step. Step `it` is started speculatively (new u into
+// registers and the LDS exchange buffer it&1) before the stop decision for
+// step it-1 is taken from the reductions written at the end of step it-1;
+// on a stop the speculative u is discarded. Everything with a side effect
+// outside registers (well alignment, chunk refill, thermal draws) happens
+// after the decision.
+//
+// Arithmetic: per particle, exactly the reference's expression order without
+// FMA; the StopList is evaluated with trailing counters equivalent to
+// GooseFEM's ring of `niter_tol` residuals initialised to +inf:
+//   descending() && all_less(tol)  <=>  dec >= n-1 && below >= n
+//   all_less(tol^2)                <=>  below2 >= n
+// Norms are reduced in a fixed order (per thread, then a butterfly per
+// wavefront, then wavefronts in order), so they are deterministic but not in
+// the reference's sequential order.
+#pragma once
+
+#include "fq_common.hpp"
+#include "fq_landscape.hpp"
+#include "fq_physics.hpp"
+
+namespace fq {
+
+constexpr int RES_MAX_WAVES = 16;
+
+template <int INT>
+constexpr bool is_chain()
+{
+    return INT == INT_NONE || INT == INT_LAPLACE1D || INT == INT_QUARTIC1D || INT == INT_QUARTICGRADIENT1D;
+}
+
+template <int PPT>
+constexpr int res_max_threads()
+{
+    return PPT <= 8 ? 1024 : 512;
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+    }
+    return x;
+}
+
+__device__ __forceinline__ long long wave_sum_i(long long x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+    }
+    return x;
+}
+
+__device__ __forceinline__ long long wave_min_i(long long x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        long long y = __shfl_xor(x, off, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+// Inclusive prefix sum within the wavefront.
+__device__ __forceinline__ int wave_incl_scan(int x, int lane)
+{
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) {
+            x += y;
+        }
+    }
+    return x;
+}
+
+// Block-wide exclusive prefix sum of `x` in thread order, plus the total.
+// Uses s_w[RES_MAX_WAVES]; contains two barriers.
+__device__ __forceinline__ int block_excl_scan(int x, int* s_w, int& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int incl = wave_incl_scan(x, lane);
+    if (lane == 63) {
+        s_w[wave] = incl;
+    }
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < nw; ++w) {
+        int v = s_w[w];
+        if (w < wave) {
+            before += v;
+        }
+        total += v;
+    }
+    __syncthreads();
+    return before + incl - x;
+}
+
+// Block-wide minimum; uses s_w[RES_MAX_WAVES]; contains two barriers.
+__device__ __forceinline__ long long block_min_i(long long x, long long* s_w)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    x = wave_min_i(x);
+    if (lane == 0) {
+        s_w[wave] = x;
+    }
+    __syncthreads();
+    long long r = s_w[0];
+    for (int w = 1; w < nw; ++w) {
+        r = s_w[w] < r ? s_w[w] : r;
+    }
+    __syncthreads();
+    return r;
+}
+
+// LDS needed by a resident launch (bytes of dynamic shared memory).
+template <int INT>
+FQ_HD size_t resident_lds_bytes(int64_t N, int threads)
+{
+    if constexpr (is_chain<INT>()) {
+        return sizeof(double) * 4 * (size_t)threads;
+    }
+    else if constexpr (INT == INT_LONGRANGE1D) {
+        return sizeof(double) * 3 * (size_t)N;
+    }
+    else {
+        return sizeof(double) * 2 * (size_t)N;
+    }
+}
+
+template <int POT, int INT, int EXT, int PPT, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ views, RunArgs ra)
+{
+    constexpr bool CHAIN = is_chain<INT>();
+    const SysView S = views[blockIdx.x];
+    RunState* rs = S.rs;
+    if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
+        return;
+    }
+
+    __shared__ double s_red_d[2][RES_MAX_WAVES][2];
+    __shared__ long long s_red_i[2][RES_MAX_WAVES][2];
+    __shared__ int s_nan[2];
+    __shared__ int s_scan_w[RES_MAX_WAVES];
+    __shared__ long long s_min_w[RES_MAX_WAVES];
+    extern __shared__ double s_dyn[];
+
+    const int t = threadIdx.x, T = blockDim.x, lane = t & 63, wave = t >> 6, nw = T >> 6;
+    const int64_t N = S.N;
+    const int64_t base = (int64_t)t * PPT;
+    const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
+    const int Tact = (int)((N + PPT - 1) / PPT);
+    const Landscape& L = S.ls;
+    const bool need_delta = ra.mode == MODE_UNTIL_EVENT || ra.mode == MODE_TRUNCATE ||
+                            (ra.mode == MODE_MINIMISE && ra.time_activity);
+
+    // LDS exchange buffers
+    double* h_first = s_dyn;         // [2][T]   chain: first slip of each thread
+    double* h_last = s_dyn + 2 * T;  // [2][T]   chain: last slip of each thread
+    double* mir = s_dyn;             // [2][N]   lattice / long range: all slips
+    double* pref = s_dyn + 2 * N;    // [N]      long range prefactor
+
+    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], fth[PPT], uprev[PPT];
+    int32_t jl[PPT];
+    int64_t dl[PPT];
+
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        u[i] = v[i] = a[i] = yl[i] = yr[i] = fth[i] = 0.0;
+        jl[i] = 0;
+        dl[i] = 0;
+        if (i < cnt) {
+            const int64_t p = base + i;
+            u[i] = S.u[p];
+            v[i] = S.v[p];
+            a[i] = S.a[p];
+            jl[i] = L.jloc[p];
+            yl[i] = L.y[p * L.nchunk + jl[i]];
+            yr[i] = L.y[p * L.nchunk + jl[i] + 1];
+            if constexpr (EXT) {
+                fth[i] = S.f_th[p];
+            }
+            if (need_delta) {
+                dl[i] = L.start[p] + jl[i] - S.i_ref[p];
+            }
+        }
+    }
+
+    auto publish = [&](int b) {
+        if constexpr (CHAIN) {
+            if (cnt > 0) {
+                h_first[b * T + t] = u[0];
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i == cnt - 1) {
+                        h_last[b * T + t] = u[i];
+                    }
+                }
+            }
+        }
+        else {
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i < cnt) {
+                    mir[(int64_t)b * N + base + i] = u[i];
+                }
+            }
+        }
+    };
+
+    if constexpr (INT == INT_LONGRANGE1D) {
+        for (int64_t q = t; q < N; q += T) {
+            pref[q] = S.pref[q];
+        }
+    }
+    publish(0);
+    if (t == 0) {
+        s_nan[0] = 0;
+        s_nan[1] = 0;
+    }
+
+    int64_t inc = rs->inc;
+    double uf = rs->u_frame;
+    int32_t below = rs->below, below2 = rs->below2, dec = rs->dec;
+    double prev = rs->prev;
+    int64_t s_n = rs->s_n;
+    int32_t act_init = rs->act_init;
+    int64_t qs_first = rs->qs_first, qs_last = rs->qs_last;
+    uint64_t rng = rs->rng_state;
+    const uint64_t rng_inc = rs->rng_inc;
+    int64_t min_next = rs->min_next;
+
+    const double dt = S.dt;
+    const double c2 = 0.5 * dt * dt;
+    const double hdt = 0.5 * dt;
+    const double meta = -S.eta;
+    const double inv_m = S.inv_m;
+    const double kf = S.k_frame;
+    const double dv = ra.v_frame * dt;
+
+    if constexpr (EXT) {
+        long long mn = 0x7fffffffffffffffLL;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                long long nx = S.next[base + i];
+                mn = nx < mn ? nx : mn;
+            }
+        }
+        min_next = block_min_i(mn, s_min_w);
+    }
+    __syncthreads();
+
+    // neighbour access of a chain particle
+    auto left_of = [&](int b) -> double { return h_last[b * T + (t == 0 ? Tact - 1 : t - 1)]; };
+    auto right_of = [&](int b) -> double { return h_first[b * T + (t == Tact - 1 ? 0 : t + 1)]; };
+
+    // interaction force of local particle i from exchange buffer b
+    auto interaction = [&](int i, int b) -> double {
+        if constexpr (INT == INT_NONE) {
+            return 0.0;
+        }
+        else if constexpr (CHAIN) {
+            double um = 0.0, up = 0.0;
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                if (q == i) {
+                    um = (q == 0) ? left_of(b) : u[q - 1];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                if (q == i) {
+                    up = (q == cnt - 1) ? right_of(b) : (q + 1 < PPT ? u[q + 1] : 0.0);
+                }
+            }
+            return chain_force<INT>(S, um, u[i], up);
+        }
+        else if constexpr (INT == INT_LONGRANGE1D) {
+            const double* U = mir + (int64_t)b * N;
+            return longrange_force(
+                S, [&](int64_t q) { return U[q]; }, [&](int64_t d) { return pref[d]; }, base + i);
+        }
+        else {
+            const double* U = mir + (int64_t)b * N;
+            const int64_t p = base + i;
+            return lattice_force<INT>(S, [&](int64_t q) { return U[q]; }, p / S.cols, p % S.cols);
+        }
+    };
+
+    int32_t status = ST_RUNNING;
+    int it;
+    for (it = 1;; ++it) {
+        const int b = it & 1;
+        // A: speculative position update of step `it`
+        int nanf = 0;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            uprev[i] = u[i];
+            if (i < cnt) {
+                u[i] = u[i] + dt * v[i] + c2 * a[i];
+                nanf |= __builtin_isnan(u[i]);
+            }
+        }
+        publish(b);
+        if (nanf) {
+            s_nan[b] = it;
+        }
+        __syncthreads();
+
+        // B: stop decision for step it-1
+        if (ra.mode != MODE_STEPS && it > 1) {
+            const int pb = (it - 1) & 1;
+            double sf = 0.0, sff = 0.0;
+            long long nch = 0, sabs = 0;
+            for (int w = 0; w < nw; ++w) {
+                sf += s_red_d[pb][w][0];
+                sff += s_red_d[pb][w][1];
+                nch += s_red_i[pb][w][0];
+                sabs += s_red_i[pb][w][1];
+            }
+            if (ra.mode == MODE_UNTIL_EVENT && nch > 0) {
+                status = ST_EVENT;
+            }
+            else {
+                const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
+                const double r = (rff != 0.0) ? rf / rff : rf;
+                below = !(r >= ra.tol) ? below + 1 : 0;
+                below2 = !(r >= ra.tol2) ? below2 + 1 : 0;
+                dec = !(r >= prev) ? dec + 1 : 0;
+                prev = r;
+                if (ra.mode == MODE_TRUNCATE || (ra.mode == MODE_MINIMISE && ra.time_activity)) {
+                    if (sabs != s_n) {
+                        if (act_init) {
+                            act_init = 0;
+                            qs_first = inc;
+                        }
+                        qs_last = inc;
+                    }
+                    s_n = sabs;
+                }
+                const int64_t nt = ra.niter_tol;
+                const bool conv = nt <= 0 || (dec >= nt - 1 && below >= nt) || below2 >= nt;
+                if (conv) {
+                    status = ST_CONVERGED;
+                }
+                else if (ra.mode == MODE_TRUNCATE) {
+                    if (ra.A_truncate > 0 && nch >= ra.A_truncate) {
+                        status = ST_TRUNCATED;
+                    }
+                    else if (ra.S_truncate > 0 && sabs >= ra.S_truncate) {
+                        status = ST_TRUNCATED;
+                    }
+                }
+            }
+        }
+        if (status != ST_RUNNING || it > ra.max_steps) {
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                u[i] = uprev[i];
+            }
+            break;
+        }
+        if (s_nan[b] == it) {
+            status = ST_NAN;
+            inc += 1;
+            break;
+        }
+
+        // C: complete step `it`
+        inc += 1;
+        if (ra.flow) {
+            uf = uf + dv;
+        }
+        if constexpr (EXT) {
+            if (inc >= min_next) {
+                int due = 0;
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i < cnt && inc >= S.next[base + i]) {
+                        ++due;
+                    }
+                }
+                int total = 0;
+                int rank = block_excl_scan(due, s_scan_w, total);
+                long long mn = 0x7fffffffffffffffLL;
+                if (due > 0) {
+                    uint64_t st = pcg_advance(rng, rng_inc, 2 * (int64_t)rank);
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        if (i < cnt) {
+                            const int64_t p = base + i;
+                            int64_t nx = S.next[p];
+                            if (inc >= nx) {
+                                fth[i] = draw_normal(st, rng_inc, S.mean, S.stddev);
+                                nx += S.dinc[p];
+                                S.next[p] = nx;
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i < cnt) {
+                        long long nx = S.next[base + i];
+                        mn = nx < mn ? nx : mn;
+                    }
+                }
+                rng = pcg_advance(rng, rng_inc, 2 * (int64_t)total);
+                min_next = block_min_i(mn, s_min_w);
+            }
+        }
+
+        double sf = 0.0, sff = 0.0;
+        long long nch = 0, sabs = 0;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t p = base + i;
+                dl[i] += align_particle(L, p, u[i], yl[i], yr[i], jl[i]);
+                const double fi = interaction(i, b);
+                const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
+                const double ff = kf * (uf - u[i]);
+                const double sum3 = ff + fp + fi;
+                const double vn = v[i], an = a[i];
+                double vv = vn + dt * an;
+                double f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                double aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                aa = f * inv_m;
+                v[i] = vv;
+                a[i] = aa;
+                sf += f * f;
+                sff += ff * ff;
+                nch += dl[i] != 0;
+                sabs += dl[i] < 0 ? -dl[i] : dl[i];
+            }
+        }
+        if (ra.mode != MODE_STEPS) {
+            sf = wave_sum(sf);
+            sff = wave_sum(sff);
+            nch = wave_sum_i(nch);
+            sabs = wave_sum_i(sabs);
+            if (lane == 0) {
+                s_red_d[b][wave][0] = sf;
+                s_red_d[b][wave][1] = sff;
+                s_red_i[b][wave][0] = nch;
+                s_red_i[b][wave][1] = sabs;
+            }
+        }
+    }
+
+    const int done = it - 1; // completed steps
+    if (status == ST_CONVERGED) {
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            v[i] = 0.0;
+            a[i] = 0.0;
+        }
+    }
+    if (status == ST_RUNNING) {
+        status = ST_BUDGET;
+    }
+
+    // write back the state and the forces of the final configuration
+    const int fb = done & 1;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        if (i < cnt) {
+            const int64_t p = base + i;
+            S.u[p] = u[i];
+            S.v[p] = v[i];
+            S.a[p] = a[i];
+            L.jloc[p] = jl[i];
+            if (status != ST_NAN) {
+                const double fi = interaction(i, fb);
+                const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
+                const double ff = kf * (uf - u[i]);
+                const double fd = meta * v[i];
+                S.f_int[p] = fi;
+                S.f_pot[p] = fp;
+                S.f_frame[p] = ff;
+                S.f_damp[p] = fd;
+                S.f[p] = EXT ? ff + fp + fi + fd + fth[i] : ff + fp + fi + fd;
+            }
+            if constexpr (EXT) {
+                S.f_th[p] = fth[i];
+            }
+        }
+    }
+    if (t == 0) {
+        rs->inc = inc;
+        rs->u_frame = uf;
+        rs->steps += done;
+        rs->status = status;
+        rs->below = below;
+        rs->below2 = below2;
+        rs->dec = dec;
+        rs->prev = prev;
+        rs->s_n = s_n;
+        rs->act_init = act_init;
+        rs->qs_first = qs_first;
+        rs->qs_last = qs_last;
+        rs->rng_state = rng;
+        rs->min_next = min_next;
+    }
+}
+
+} // namespace fq

@@ -1,0 +1,102 @@
+// Launchers of the service kernels (fq_service.hpp).
+#include "fq_launch.hpp"
+#include "fq_service.hpp"
+
+namespace fq {
+
+hipError_t launch_landscape_init(const SysView* views, int nsys, int64_t maxN, hipStream_t stream)
+{
+    dim3 grid((unsigned)((maxN + SVC_THREADS - 1) / SVC_THREADS), (unsigned)nsys);
+    hipLaunchKernelGGL(k_landscape_init, grid, dim3(SVC_THREADS), 0, stream, views);
+    return hipGetLastError();
+}
+
+hipError_t launch_restore(const SysView* views, int64_t N, const uint64_t* state, const double* value,
+                          const int64_t* index, hipStream_t stream)
+{
+    dim3 grid((unsigned)((N + SVC_THREADS - 1) / SVC_THREADS), 1);
+    hipLaunchKernelGGL(k_restore, grid, dim3(SVC_THREADS), 0, stream, views, state, value, index);
+    return hipGetLastError();
+}
+
+hipError_t launch_trigger(const SysView* views, int64_t p, double eps, int32_t direction, int32_t do_align,
+                          hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_trigger, dim3(1), dim3(1), 0, stream, views, p, eps, direction, do_align);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_iref(const SysView* views, int nsys, int64_t maxN, hipStream_t stream)
+{
+    dim3 grid((unsigned)((maxN + SVC_THREADS - 1) / SVC_THREADS), (unsigned)nsys);
+    hipLaunchKernelGGL(k_set_iref, grid, dim3(SVC_THREADS), 0, stream, views);
+    return hipGetLastError();
+}
+
+#define FQ_FORCES(P, I, E)                                                                                     \
+    if (pot == P && inter == I && ext == E) {                                                                  \
+        hipLaunchKernelGGL((k_forces<P, I, E>), dim3(nsys), dim3(SVC_THREADS), 0, stream, views, flags);       \
+        return true;                                                                                           \
+    }
+
+bool launch_forces(int pot, int inter, int ext, const SysView* views, int nsys, int32_t flags, hipStream_t stream)
+{
+    FQ_FORCES(POT_CUSPY, INT_LAPLACE1D, 0)
+    FQ_FORCES(POT_CUSPY, INT_LAPLACE1D, 1)
+    FQ_FORCES(POT_CUSPY, INT_QUARTIC1D, 0)
+    FQ_FORCES(POT_CUSPY, INT_QUARTIC1D, 1)
+    FQ_FORCES(POT_CUSPY, INT_QUARTICGRADIENT1D, 0)
+    FQ_FORCES(POT_CUSPY, INT_NONE, 0)
+    FQ_FORCES(POT_CUSPY, INT_NONE, 1)
+    FQ_FORCES(POT_CUSPY, INT_LONGRANGE1D, 0)
+    FQ_FORCES(POT_CUSPY, INT_LONGRANGE1D, 1)
+    FQ_FORCES(POT_CUSPY, INT_LAPLACE2D, 0)
+    FQ_FORCES(POT_CUSPY, INT_QUARTICGRADIENT2D, 0)
+    FQ_FORCES(POT_SEMISMOOTH, INT_LAPLACE1D, 0)
+    FQ_FORCES(POT_SEMISMOOTH, INT_QUARTIC1D, 0)
+    FQ_FORCES(POT_SEMISMOOTH, INT_NONE, 0)
+    FQ_FORCES(POT_SMOOTH, INT_LAPLACE1D, 0)
+    FQ_FORCES(POT_SMOOTH, INT_QUARTIC1D, 0)
+    FQ_FORCES(POT_SMOOTH, INT_NONE, 0)
+    return false;
+}
+
+bool launch_max_uniform(int pot, const SysView* views, int nsys, int32_t direction, hipStream_t stream)
+{
+    if (pot == POT_CUSPY) {
+        hipLaunchKernelGGL((k_max_uniform<POT_CUSPY>), dim3(nsys), dim3(SVC_THREADS), 0, stream, views, direction);
+        return true;
+    }
+    if (pot == POT_SEMISMOOTH) {
+        hipLaunchKernelGGL((k_max_uniform<POT_SEMISMOOTH>), dim3(nsys), dim3(SVC_THREADS), 0, stream, views,
+                           direction);
+        return true;
+    }
+    return false;
+}
+
+bool launch_nopassing(int ppt, int threads, size_t lds, const SysView* views, int nsys, const RunArgs& ra,
+                      hipStream_t stream)
+{
+    switch (ppt) {
+        case 1:
+            hipLaunchKernelGGL((k_nopassing<1>), dim3(nsys), dim3(threads), lds, stream, views, ra);
+            return true;
+        case 2:
+            hipLaunchKernelGGL((k_nopassing<2>), dim3(nsys), dim3(threads), lds, stream, views, ra);
+            return true;
+        case 4:
+            hipLaunchKernelGGL((k_nopassing<4>), dim3(nsys), dim3(threads), lds, stream, views, ra);
+            return true;
+        case 8:
+            hipLaunchKernelGGL((k_nopassing<8>), dim3(nsys), dim3(threads), lds, stream, views, ra);
+            return true;
+        case 16:
+            hipLaunchKernelGGL((k_nopassing<16>), dim3(nsys), dim3(threads), lds, stream, views, ra);
+            return true;
+        default:
+            return false;
+    }
+}
+
+} // namespace fq

@@ -1,0 +1,230 @@
+/*
+ * fqsb.h -- C ABI of the MI355X (gfx950) implementation of the
+ * FrictionQPotSpringBlock dynamics hot path (library libfqsb.so).
+ *
+ * Plain pointers and sizes only. Every array argument is host memory of the
+ * system's size N = prod(shape) (flat, row-major for Line2d), copied in or out
+ * by the call; the library owns all device memory. Every function returns 0 on
+ * success and non-zero on failure, with the message in fqsb_last_error()
+ * (thread-local). Messages of errors that the reference raises are the
+ * reference's own texts ("NaN entries found", "No convergence found",
+ * "Operation not possible.", "Minimisation not implementated", "Unknown
+ * distribution: X").
+ *
+ * The reference binds these operations through pybind11 (python/main.cpp)
+ * onto the C++ class template detail::System
+ * (include/FrictionQPotSpringBlock/detail.h:1046-2051); each declaration
+ * below cites the member it replaces. Not thread-safe per handle.
+ */
+#ifndef FQSB_H
+#define FQSB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* potential policy: detail::Cuspy detail.h:113, SemiSmooth detail.h:212, Smooth detail.h:356 */
+enum { FQSB_POT_CUSPY = 0, FQSB_POT_SEMISMOOTH = 1, FQSB_POT_SMOOTH = 2 };
+
+/* interaction policy: Laplace1d detail.h:446, Quartic1d detail.h:757,
+ * QuarticGradient1d detail.h:616, LongRange1d detail.h:812, Laplace2d
+ * detail.h:524, QuarticGradient2d detail.h:658; NONE = no interactions */
+enum {
+    FQSB_INT_NONE = 0,
+    FQSB_INT_LAPLACE1D = 1,
+    FQSB_INT_QUARTIC1D = 2,
+    FQSB_INT_QUARTICGRADIENT1D = 3,
+    FQSB_INT_LONGRANGE1D = 4,
+    FQSB_INT_LAPLACE2D = 5,
+    FQSB_INT_QUARTICGRADIENT2D = 6
+};
+
+/* external policy: RandomNormalForcing detail.h:881 */
+enum { FQSB_EXT_NONE = 0, FQSB_EXT_RANDOMFORCING = 1 };
+
+/* minimisation tag: inertial (default), Overdamped (no-passing, detail.h:1005),
+ * None (detail.h:1015: minimise raises) */
+enum { FQSB_MIN_INERTIAL = 0, FQSB_MIN_OVERDAMPED = 1, FQSB_MIN_NONE = 2 };
+
+/* landscape increment distribution: string_to_distribution detail.h:31-66 */
+enum {
+    FQSB_DIST_RANDOM = 0,
+    FQSB_DIST_DELTA = 1,
+    FQSB_DIST_EXPONENTIAL = 2,
+    FQSB_DIST_POWER = 3,
+    FQSB_DIST_GAMMA = 4,
+    FQSB_DIST_PARETO = 5,
+    FQSB_DIST_WEIBULL = 6,
+    FQSB_DIST_NORMAL = 7
+};
+
+/* array fields: System::u/v/a/f/f_potential/f_interactions/f_frame/f_damping
+ * detail.h:1402-1468, RandomNormalForcing::f_thermal detail.h:967 */
+enum {
+    FQSB_U = 0,
+    FQSB_V = 1,
+    FQSB_A = 2,
+    FQSB_F = 3,
+    FQSB_F_POTENTIAL = 4,
+    FQSB_F_INTERACTIONS = 5,
+    FQSB_F_FRAME = 6,
+    FQSB_F_DAMPING = 7,
+    FQSB_F_THERMAL = 8
+};
+
+/* Description of a system: the arguments of the reference's constructors
+ * (Line1d.h:134-161, 199-225, 283-318, 348-376, 394-421, 451-479, 508-555,
+ * 585-613, 643-671; Line2d.h:88-115, 133-161). */
+typedef struct fqsb_desc {
+    int32_t rank;         /* 1: Line1d (shape [rows]); 2: Line2d (shape [rows, cols]) */
+    int32_t potential;    /* FQSB_POT_* */
+    int32_t interactions; /* FQSB_INT_* */
+    int32_t external;     /* FQSB_EXT_* */
+    int32_t minimisation; /* FQSB_MIN_* */
+    int32_t distribution; /* FQSB_DIST_* */
+    uint64_t rows;
+    uint64_t cols; /* 1 for rank 1 */
+    uint64_t seed; /* particle p uses pcg32 stream (seed + p, 0), Line1d.h:150 */
+    uint64_t nchunk;
+    double m, eta, mu, kappa, k_frame, dt;
+    double k_interactions; /* Laplace1d/2d, LongRange1d */
+    double a1, a2;         /* Quartic1d */
+    double k2, k4;         /* QuarticGradient1d/2d */
+    double alpha;          /* LongRange1d */
+    double offset;         /* m_gen += offset, Line1d.h:157 */
+    double parameters[4];
+    int32_t nparameters;
+    int32_t pad0;
+    double mean, stddev; /* RandomNormalForcing */
+    uint64_t seed_forcing;
+    const int64_t* dinc_init; /* [N] */
+    const int64_t* dinc;      /* [N] */
+    int64_t align_buffer;     /* prrng::alignment(buffer=2, margin=30, ...), Line1d.h:155 */
+    int64_t align_margin;
+} fqsb_desc;
+
+typedef struct fqsb_system fqsb_system;
+
+/* Last error message of the calling thread. */
+const char* fqsb_last_error(void);
+
+/* Library version string. */
+const char* fqsb_version(void);
+
+/* Construct (Line1d.h:134-161 etc.: landscape, policies, initSystem
+ * detail.h:1096-1139). */
+int fqsb_create(const fqsb_desc* desc, fqsb_system** out);
+int fqsb_destroy(fqsb_system* s);
+
+/* System::size detail.h:1155 */
+uint64_t fqsb_size(const fqsb_system* s);
+
+/* Getters detail.h:1402-1468 (copy out) and setters set_u/set_v/set_a
+ * detail.h:1276-1305, RandomNormalForcing::set_f_thermal detail.h:976. */
+int fqsb_get(fqsb_system* s, int field, double* out);
+int fqsb_set(fqsb_system* s, int field, const double* in);
+
+/* inc / set_inc detail.h:1486, 1241; set_t detail.h:1231 */
+int fqsb_get_inc(const fqsb_system* s, int64_t* out);
+int fqsb_set_inc(fqsb_system* s, int64_t inc);
+int fqsb_set_t(fqsb_system* s, double t);
+
+/* u_frame / set_u_frame detail.h:1264, 1253 */
+int fqsb_get_u_frame(const fqsb_system* s, double* out);
+int fqsb_set_u_frame(fqsb_system* s, double u_frame);
+
+/* quasistaticActivityFirst/Last detail.h:1802-1818 */
+int fqsb_quasistatic_activity(const fqsb_system* s, int64_t* first, int64_t* last);
+
+/* residual detail.h:1512, temperature detail.h:1500 */
+int fqsb_residual(fqsb_system* s, double* out);
+int fqsb_temperature(fqsb_system* s, double* out);
+
+/* refresh detail.h:1310, quench detail.h:1527 */
+int fqsb_refresh(fqsb_system* s);
+int fqsb_quench(fqsb_system* s);
+
+/* timeStep detail.h:1539, timeSteps detail.h:1577, flowSteps detail.h:1637 */
+int fqsb_time_step(fqsb_system* s);
+int fqsb_time_steps(fqsb_system* s, uint64_t n);
+int fqsb_flow_steps(fqsb_system* s, uint64_t n, double v_frame);
+
+/* timeStepsUntilEvent detail.h:1595 */
+int fqsb_time_steps_until_event(fqsb_system* s, double tol, uint64_t niter_tol, uint64_t max_iter, uint64_t* ret);
+
+/* minimise detail.h:1676 (inertial, or no-passing for FQSB_MIN_OVERDAMPED) */
+int fqsb_minimise(fqsb_system* s, double tol, uint64_t niter_tol, uint64_t max_iter, int time_activity,
+                  int max_iter_is_error, uint64_t* ret);
+
+/* minimise_truncate detail.h:1833 */
+int fqsb_minimise_truncate(fqsb_system* s, const int64_t* i_n, uint64_t A_truncate, uint64_t S_truncate, double tol,
+                           uint64_t niter_tol, uint64_t max_iter, int time_activity, int max_iter_is_error,
+                           uint64_t* ret);
+
+/* maxUniformDisplacement detail.h:1901, eventDrivenStep detail.h:1933,
+ * trigger detail.h:1972, advanceToFixedForce detail.h:1988 */
+int fqsb_max_uniform_displacement(fqsb_system* s, int direction, double* ret);
+int fqsb_event_driven_step(fqsb_system* s, double eps, int kick, int direction, double* ret);
+int fqsb_trigger(fqsb_system* s, uint64_t p, double eps, int direction);
+int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plastic);
+
+/* The chunk of yield positions (System::chunk detail.h:1146; the prrng
+ * pcg32_tensor_cumsum members used by the reference and its tests:
+ * data(), index_at_align(), chunk_index_at_align(), start(), state_at(),
+ * restore(), test_Line1d.py:309-341). data is [N, nchunk]. */
+uint64_t fqsb_chunk_size(const fqsb_system* s);
+int fqsb_chunk_data(fqsb_system* s, double* out);
+int fqsb_chunk_index_at_align(fqsb_system* s, int64_t* out);
+int fqsb_chunk_chunk_index_at_align(fqsb_system* s, int64_t* out);
+int fqsb_chunk_start(fqsb_system* s, int64_t* out);
+int fqsb_chunk_state_at(fqsb_system* s, const int64_t* index, uint64_t* out);
+int fqsb_chunk_restore(fqsb_system* s, const uint64_t* state, const double* value, const int64_t* index);
+
+/* RandomNormalForcing state/next detail.h:949-999 */
+int fqsb_thermal_state(const fqsb_system* s, uint64_t* out);
+int fqsb_thermal_set_state(fqsb_system* s, uint64_t state);
+int fqsb_thermal_next(fqsb_system* s, int64_t* out);
+int fqsb_thermal_set_next(fqsb_system* s, const int64_t* in);
+
+/* ------------------------------------------------------------------------
+ * Batched form (no reference counterpart: the reference runs one system per
+ * object). R replicas of one description; replica r uses landscape seed
+ * desc->seed + r * seed_stride, so with seed_stride = 1 replica r equals the
+ * single system constructed with seed = desc->seed + r. All replicas advance
+ * together in one resident launch (one workgroup per replica).
+ * ---------------------------------------------------------------------- */
+typedef struct fqsb_batch fqsb_batch;
+
+int fqsb_batch_create(const fqsb_desc* desc, uint64_t replicas, uint64_t seed_stride, fqsb_batch** out);
+int fqsb_batch_destroy(fqsb_batch* b);
+uint64_t fqsb_batch_replicas(const fqsb_batch* b);
+/* Choose the resident launch shape (particles per thread, threads per
+ * workgroup); 0, 0 restores the automatic choice. */
+int fqsb_batch_set_launch(fqsb_batch* b, int ppt, int threads);
+int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads);
+/* flowSteps / timeSteps on all replicas, launched as resident blocks of
+ * `block` steps (0: automatic). If ms is non-NULL it receives the time
+ * between HIP events recorded on the launch stream around the launches. */
+int fqsb_batch_flow_steps(fqsb_batch* b, uint64_t n, double v_frame, uint64_t block, float* ms);
+int fqsb_batch_time_steps(fqsb_batch* b, uint64_t n, uint64_t block, float* ms);
+/* minimise on all replicas; ret[r] as System::minimise. */
+int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                        uint64_t* ret);
+/* eventDrivenStep on all replicas (each with its own displacement); ret[r]. */
+int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direction, double* ret);
+/* Arrays of all replicas, [R, N]; scalars [R]. */
+int fqsb_batch_get(fqsb_batch* b, int field, double* out);
+int fqsb_batch_set(fqsb_batch* b, int field, const double* in);
+int fqsb_batch_get_u_frame(fqsb_batch* b, double* out);
+int fqsb_batch_get_inc(fqsb_batch* b, int64_t* out);
+int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out);
+/* The HIP stream (hipStream_t) the batch launches on. */
+void* fqsb_batch_stream(fqsb_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
