@@ -1,0 +1,372 @@
+"""Batched replicas: many systems of one description advanced in one launch.
+
+No reference counterpart (the reference advances one system per object); this
+is the batched form of the same operations, used for ensembles of replicas.
+Replica r is built from landscape seed ``seed + r * seed_stride``; with
+``seed_stride=1`` it equals the single system constructed with that seed
+(Line1d.h:150: particle p of a system draws from stream ``seed + p``).
+
+Thin ctypes layer over the C ABI of ``libfqsb.so`` (include/fqsb.h, the
+``fqsb_batch_*`` functions); all work runs in the HIP library.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_here, "libfqsb.so")
+
+POTENTIAL = {"Cuspy": 0, "SemiSmooth": 1, "Smooth": 2}
+INTERACTIONS = {
+    None: 0,
+    "Laplace1d": 1,
+    "Quartic1d": 2,
+    "QuarticGradient1d": 3,
+    "LongRange1d": 4,
+    "Laplace2d": 5,
+    "QuarticGradient2d": 6,
+}
+DISTRIBUTION = {
+    "random": 0,
+    "delta": 1,
+    "exponential": 2,
+    "power": 3,
+    "gamma": 4,
+    "pareto": 5,
+    "weibull": 6,
+    "normal": 7,
+}
+FIELD = {
+    "u": 0,
+    "v": 1,
+    "a": 2,
+    "f": 3,
+    "f_potential": 4,
+    "f_interactions": 5,
+    "f_frame": 6,
+    "f_damping": 7,
+    "f_thermal": 8,
+}
+
+
+class Desc(C.Structure):
+    """Mirror of ``fqsb_desc`` (include/fqsb.h)."""
+
+    _fields_ = [
+        ("rank", C.c_int32),
+        ("potential", C.c_int32),
+        ("interactions", C.c_int32),
+        ("external", C.c_int32),
+        ("minimisation", C.c_int32),
+        ("distribution", C.c_int32),
+        ("rows", C.c_uint64),
+        ("cols", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("nchunk", C.c_uint64),
+        ("m", C.c_double),
+        ("eta", C.c_double),
+        ("mu", C.c_double),
+        ("kappa", C.c_double),
+        ("k_frame", C.c_double),
+        ("dt", C.c_double),
+        ("k_interactions", C.c_double),
+        ("a1", C.c_double),
+        ("a2", C.c_double),
+        ("k2", C.c_double),
+        ("k4", C.c_double),
+        ("alpha", C.c_double),
+        ("offset", C.c_double),
+        ("parameters", C.c_double * 4),
+        ("nparameters", C.c_int32),
+        ("pad0", C.c_int32),
+        ("mean", C.c_double),
+        ("stddev", C.c_double),
+        ("seed_forcing", C.c_uint64),
+        ("dinc_init", C.POINTER(C.c_int64)),
+        ("dinc", C.POINTER(C.c_int64)),
+        ("align_buffer", C.c_int64),
+        ("align_margin", C.c_int64),
+    ]
+
+
+_P = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+_up = C.POINTER(C.c_uint64)
+_fp = C.POINTER(C.c_float)
+
+# (restype, argtypes) of every function declared in include/fqsb.h
+SIGNATURES = {
+    "fqsb_last_error": (C.c_char_p, []),
+    "fqsb_version": (C.c_char_p, []),
+    "fqsb_create": (C.c_int, [C.POINTER(Desc), C.POINTER(_P)]),
+    "fqsb_destroy": (C.c_int, [_P]),
+    "fqsb_size": (C.c_uint64, [_P]),
+    "fqsb_get": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_set": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_get_inc": (C.c_int, [_P, _ip]),
+    "fqsb_set_inc": (C.c_int, [_P, C.c_int64]),
+    "fqsb_set_t": (C.c_int, [_P, C.c_double]),
+    "fqsb_get_u_frame": (C.c_int, [_P, _dp]),
+    "fqsb_set_u_frame": (C.c_int, [_P, C.c_double]),
+    "fqsb_quasistatic_activity": (C.c_int, [_P, _ip, _ip]),
+    "fqsb_residual": (C.c_int, [_P, _dp]),
+    "fqsb_temperature": (C.c_int, [_P, _dp]),
+    "fqsb_refresh": (C.c_int, [_P]),
+    "fqsb_quench": (C.c_int, [_P]),
+    "fqsb_time_step": (C.c_int, [_P]),
+    "fqsb_time_steps": (C.c_int, [_P, C.c_uint64]),
+    "fqsb_flow_steps": (C.c_int, [_P, C.c_uint64, C.c_double]),
+    "fqsb_time_steps_until_event": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, _up]),
+    "fqsb_minimise": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, C.c_int, _up]),
+    "fqsb_minimise_truncate": (
+        C.c_int,
+        [_P, _ip, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.c_int, C.c_int, _up],
+    ),
+    "fqsb_max_uniform_displacement": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_event_driven_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _dp]),
+    "fqsb_trigger": (C.c_int, [_P, C.c_uint64, C.c_double, C.c_int]),
+    "fqsb_advance_to_fixed_force": (C.c_int, [_P, C.c_double, C.c_int]),
+    "fqsb_chunk_size": (C.c_uint64, [_P]),
+    "fqsb_chunk_data": (C.c_int, [_P, _dp]),
+    "fqsb_chunk_index_at_align": (C.c_int, [_P, _ip]),
+    "fqsb_chunk_chunk_index_at_align": (C.c_int, [_P, _ip]),
+    "fqsb_chunk_start": (C.c_int, [_P, _ip]),
+    "fqsb_chunk_state_at": (C.c_int, [_P, _ip, _up]),
+    "fqsb_chunk_restore": (C.c_int, [_P, _up, _dp, _ip]),
+    "fqsb_thermal_state": (C.c_int, [_P, _up]),
+    "fqsb_thermal_set_state": (C.c_int, [_P, C.c_uint64]),
+    "fqsb_thermal_next": (C.c_int, [_P, _ip]),
+    "fqsb_thermal_set_next": (C.c_int, [_P, _ip]),
+    "fqsb_batch_create": (C.c_int, [C.POINTER(Desc), C.c_uint64, C.c_uint64, C.POINTER(_P)]),
+    "fqsb_batch_destroy": (C.c_int, [_P]),
+    "fqsb_batch_replicas": (C.c_uint64, [_P]),
+    "fqsb_batch_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fqsb_batch_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fqsb_batch_flow_steps": (C.c_int, [_P, C.c_uint64, C.c_double, C.c_uint64, _fp]),
+    "fqsb_batch_time_steps": (C.c_int, [_P, C.c_uint64, C.c_uint64, _fp]),
+    "fqsb_batch_minimise": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up]),
+    "fqsb_batch_event_driven_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _dp]),
+    "fqsb_batch_get": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_batch_set": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_batch_get_u_frame": (C.c_int, [_P, _dp]),
+    "fqsb_batch_get_inc": (C.c_int, [_P, _ip]),
+    "fqsb_batch_chunk_index_at_align": (C.c_int, [_P, _ip]),
+    "fqsb_batch_stream": (C.c_void_p, [_P]),
+}
+
+_lib = None
+
+
+def capi():
+    """The C ABI of libfqsb.so with argument types declared."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(f"{LIBRARY_PATH} is not built; run `python -m frictionqpotspringblock_amd.build_ext`")
+        L = C.CDLL(LIBRARY_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(capi().fqsb_last_error().decode())
+
+
+def make_desc(
+    *,
+    shape,
+    seed,
+    distribution,
+    parameters,
+    potential="Cuspy",
+    interactions="Laplace1d",
+    m=1.0,
+    eta=0.0,
+    mu=1.0,
+    kappa=0.0,
+    k_interactions=0.0,
+    k_frame=0.0,
+    dt=0.0,
+    a1=0.0,
+    a2=0.0,
+    k2=0.0,
+    k4=0.0,
+    alpha=0.0,
+    offset=-100.0,
+    nchunk=5000,
+):
+    """Build a ``fqsb_desc`` from the reference's constructor arguments."""
+    if distribution not in DISTRIBUTION:
+        raise RuntimeError("Unknown distribution: " + str(distribution))
+    shape = [int(s) for s in shape]
+    d = Desc()
+    d.rank = len(shape)
+    d.potential = POTENTIAL[potential]
+    d.interactions = INTERACTIONS[interactions]
+    d.distribution = DISTRIBUTION[distribution]
+    d.rows = shape[0]
+    d.cols = shape[1] if len(shape) == 2 else 1
+    d.seed = int(seed)
+    d.nchunk = int(nchunk)
+    d.m, d.eta, d.mu, d.kappa = float(m), float(eta), float(mu), float(kappa)
+    d.k_frame, d.dt, d.k_interactions = float(k_frame), float(dt), float(k_interactions)
+    d.a1, d.a2, d.k2, d.k4, d.alpha = float(a1), float(a2), float(k2), float(k4), float(alpha)
+    d.offset = float(offset)
+    par = list(parameters)
+    if len(par) > 4:
+        raise RuntimeError("Too many parameters for distribution")
+    for i, p in enumerate(par):
+        d.parameters[i] = float(p)
+    d.nparameters = len(par)
+    d.align_buffer = 2
+    d.align_margin = 30
+    return d
+
+
+class Batch:
+    """R replicas of one Line1d/Line2d system advanced together on the GPU."""
+
+    def __init__(self, replicas, seed_stride=1, **kw):
+        self._desc = make_desc(**kw)
+        self._shape = [int(s) for s in kw["shape"]]
+        self._N = int(np.prod(self._shape))
+        self._R = int(replicas)
+        h = C.c_void_p()
+        _check(capi().fqsb_batch_create(C.byref(self._desc), self._R, int(seed_stride), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.fqsb_batch_destroy(h)
+            self._h = None
+
+    @property
+    def replicas(self):
+        return self._R
+
+    @property
+    def size(self):
+        return self._N
+
+    @property
+    def shape(self):
+        return list(self._shape)
+
+    @property
+    def stream(self):
+        """The hipStream_t (as an integer) the batch launches on."""
+        return capi().fqsb_batch_stream(self._h)
+
+    def set_launch(self, ppt=0, threads=0):
+        _check(capi().fqsb_batch_set_launch(self._h, int(ppt), int(threads)))
+
+    @property
+    def launch(self):
+        p, t = C.c_int(), C.c_int()
+        _check(capi().fqsb_batch_get_launch(self._h, C.byref(p), C.byref(t)))
+        return p.value, t.value
+
+    # launch shapes (particles per thread, threads per workgroup) the autotuner tries
+    CANDIDATES = [(1, 1024), (2, 1024), (4, 1024), (8, 512), (8, 1024), (16, 256), (16, 512), (32, 128), (32, 256)]
+
+    def autotune(self, v_frame=0.0, steps=500, candidates=None):
+        """Time one resident block per launch shape and keep the fastest.
+
+        The trial steps advance the batch like flowSteps(steps, v_frame) per
+        candidate; use it during warm-up. Returns {(ppt, threads): ms}.
+        """
+        timings = {}
+        for ppt, threads in candidates or self.CANDIDATES:
+            need = -(-self._N // ppt)  # threads that own particles
+            if need > threads or need <= threads // 2:
+                continue
+            try:
+                self.set_launch(ppt, threads)
+            except RuntimeError:
+                continue
+            timings[(ppt, threads)] = self.flowSteps(steps, v_frame, block=steps, timed=True)
+        if not timings:
+            self.set_launch(0, 0)
+            return timings
+        best = min(timings, key=timings.get)
+        self.set_launch(*best)
+        return timings
+
+    def flowSteps(self, n, v_frame, block=0, timed=False):
+        ms = C.c_float(0.0)
+        _check(
+            capi().fqsb_batch_flow_steps(
+                self._h, int(n), float(v_frame), int(block), C.byref(ms) if timed else None
+            )
+        )
+        return ms.value if timed else None
+
+    def timeSteps(self, n, block=0, timed=False):
+        ms = C.c_float(0.0)
+        _check(capi().fqsb_batch_time_steps(self._h, int(n), int(block), C.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def minimise(self, tol=1e-5, niter_tol=10, max_iter=int(1e9), max_iter_is_error=True):
+        ret = np.zeros(self._R, dtype=np.uint64)
+        _check(
+            capi().fqsb_batch_minimise(
+                self._h, tol, niter_tol, int(max_iter), int(max_iter_is_error), ret.ctypes.data_as(_up)
+            )
+        )
+        return ret
+
+    def eventDrivenStep(self, eps, kick, direction=1):
+        ret = np.zeros(self._R)
+        _check(
+            capi().fqsb_batch_event_driven_step(
+                self._h, float(eps), int(bool(kick)), int(direction), ret.ctypes.data_as(_dp)
+            )
+        )
+        return ret
+
+    def _get(self, name):
+        out = np.empty(self._R * self._N)
+        _check(capi().fqsb_batch_get(self._h, FIELD[name], out.ctypes.data_as(_dp)))
+        return out.reshape([self._R] + self._shape)
+
+    def _set(self, name, value):
+        v = np.ascontiguousarray(np.broadcast_to(value, [self._R] + self._shape), dtype=float).ravel()
+        _check(capi().fqsb_batch_set(self._h, FIELD[name], v.ctypes.data_as(_dp)))
+
+    u = property(lambda self: self._get("u"), lambda self, v: self._set("u", v))
+    v = property(lambda self: self._get("v"), lambda self, v: self._set("v", v))
+    a = property(lambda self: self._get("a"), lambda self, v: self._set("a", v))
+    f = property(lambda self: self._get("f"))
+    f_potential = property(lambda self: self._get("f_potential"))
+    f_interactions = property(lambda self: self._get("f_interactions"))
+    f_frame = property(lambda self: self._get("f_frame"))
+    f_damping = property(lambda self: self._get("f_damping"))
+
+    @property
+    def u_frame(self):
+        out = np.empty(self._R)
+        _check(capi().fqsb_batch_get_u_frame(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    @property
+    def inc(self):
+        out = np.empty(self._R, dtype=np.int64)
+        _check(capi().fqsb_batch_get_inc(self._h, out.ctypes.data_as(_ip)))
+        return out
+
+    @property
+    def index_at_align(self):
+        out = np.empty(self._R * self._N, dtype=np.int64)
+        _check(capi().fqsb_batch_chunk_index_at_align(self._h, out.ctypes.data_as(_ip)))
+        return out.reshape([self._R] + self._shape)

@@ -192,6 +192,12 @@ FQ_HD uint64_t pcg_advance(uint64_t state, uint64_t inc, int64_t delta)
     return acc_mult * state + acc_plus;
 }
 
+// One step back: the LCG state before `state` (inverse multiplier mod 2^64).
+FQ_HD uint64_t pcg_step_back(uint64_t state, uint64_t inc)
+{
+    return (state - inc) * 0xc097ef87329e28a5ULL;
+}
+
 FQ_HD double u32_to_double(uint32_t x)
 {
     uint64_t b = ((uint64_t)x << 20) | 0x3ff0000000000000ULL;

@@ -18,6 +18,8 @@
 
 namespace fq {
 
+constexpr int64_t MAX_SEARCH_WELLS = int64_t(1) << 20;
+
 // Move particle p's window so that it starts at global index ns.
 __device__ __noinline__ void chunk_shift_to(const Landscape& L, int64_t p, int64_t ns)
 {
@@ -55,7 +57,7 @@ __device__ __noinline__ void chunk_shift_to(const Landscape& L, int64_t p, int64
             }
         }
         for (int64_t g = s; g > ns; --g) {
-            st = pcg_advance(st, L.inc, -1);
+            st = pcg_step_back(st, L.inc);
             uint64_t tmp = st;
             double dg = draw_increment(L, tmp);
             val = val - dg;
@@ -81,11 +83,24 @@ __device__ __noinline__ int32_t chunk_align_slow(const Landscape& L, int64_t p, 
         return j0;
     }
     const int64_t b = L.buffer, m = L.margin;
+    // A slip further than MAX_SEARCH_WELLS wells from the window is not
+    // searched (bounded work per call); the particle is left at the window edge.
+    int64_t budget = MAX_SEARCH_WELLS;
     while (!(u < y[n - 1])) {
+        if (budget <= 0) {
+            L.jloc[p] = (int32_t)(n - 2);
+            return (int32_t)(n - 2);
+        }
+        budget -= n - 1 - m;
         chunk_shift_to(L, p, L.start[p] + (n - 1 - m));
         j = m;
     }
     while (u < y[0]) {
+        if (budget <= 0) {
+            L.jloc[p] = 0;
+            return 0;
+        }
+        budget -= n - 1 - m;
         chunk_shift_to(L, p, L.start[p] - (n - 1 - m));
         j = n - 1 - m;
     }

@@ -345,7 +345,9 @@ __global__ void __launch_bounds__(res_max_threads<PPT>()) k_nopassing(const SysV
                 }
                 const double uneigh = um + up;
                 double umin, un;
-                while (true) {
+                // the well sequence is monotone (the minimum moves with the
+                // well), so this terminates; the cap only bounds the work
+                for (int guard = 0; guard < (1 << 20); ++guard) {
                     umin = 0.5 * (yl[i] + yr[i]);
                     un = (k * uneigh + kf * uf + mu * umin) / (2 * k + kf + mu);
                     const int32_t jold = jl[i];

@@ -567,7 +567,7 @@ static void chunk_shift_to(fq_chunk* c, uint64_t p, int64_t ns)
         }
         for (int64_t g = s; g > ns; --g) {
             /* y[g-1] = y[g] - d_g, d_g drawn from the state that produced it */
-            st = fqo_pcg32_advance(st, c->inc, -1);
+            st = (st - c->inc) * 0xc097ef87329e28a5ULL; /* one step back */
             uint64_t tmp = st;
             double dg = draw_one(&c->tr, &tmp, c->inc);
             val = val - dg;
@@ -596,11 +596,23 @@ static void chunk_align_p(fq_chunk* c, uint64_t p, double u)
         return;
     }
     const int64_t b = c->buffer, m = c->margin;
+    /* bounded search: a slip further than 2^20 wells away is not followed */
+    int64_t budget = (int64_t)1 << 20;
     while (!(u < y[n - 1])) {
+        if (budget <= 0) {
+            c->j[p] = n - 2;
+            return;
+        }
+        budget -= n - 1 - m;
         chunk_shift_to(c, p, c->start[p] + (n - 1 - m));
         j = m;
     }
     while (u < y[0]) {
+        if (budget <= 0) {
+            c->j[p] = 0;
+            return;
+        }
+        budget -= n - 1 - m;
         chunk_shift_to(c, p, c->start[p] - (n - 1 - m));
         j = n - 1 - m;
     }

@@ -1,0 +1,115 @@
+"""Shared system definitions and protocols of the parity tests.
+
+``EXAMPLES`` restates the systems of the reference's example scripts
+(examples/*.py, whose outputs are the golden files converted into
+tests/golden/*.npz); ``quasistatic`` restates their event-driven protocol
+(examples/Line1d_Cuspy_Laplace.py:39-61). Both work for any module that
+exposes the reference's API: the product (frictionqpotspringblock_amd) or the
+oracle (oracle.pyoracle).
+"""
+
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+_N = 1000
+_COMMON = dict(
+    m=1.0,
+    eta=2.0 * np.sqrt(3.0) / 10.0,
+    mu=1.0,
+    k_frame=1.0 / _N,
+    dt=0.1,
+    shape=[_N],
+    seed=0,
+    distribution="random",
+    parameters=[2.0],
+    offset=-50,
+)
+
+
+def _nopassing():
+    c = dict(_COMMON)
+    for k in ("m", "eta", "dt"):
+        c.pop(k)
+    return dict(k_interactions=1.0, **c)
+
+
+def _line2d():
+    c = dict(_COMMON)
+    c["shape"] = [50, 50]
+    c["k_frame"] = 1.0 / 2500
+    return dict(k_interactions=1.0, **c)
+
+
+# name -> (submodule, class, constructor kwargs); examples/<name>.py
+EXAMPLES = {
+    "Line1d_Cuspy_Laplace": ("Line1d", "System_Cuspy_Laplace", dict(k_interactions=1.0, **_COMMON)),
+    "Line1d_Cuspy_Quartic": ("Line1d", "System_Cuspy_Quartic", dict(a1=1.0, a2=1.0, **_COMMON)),
+    "Line1d_SemiSmooth_Laplace": (
+        "Line1d",
+        "System_SemiSmooth_Laplace",
+        dict(k_interactions=1.0, kappa=1.0, **_COMMON),
+    ),
+    "Line1d_Cuspy_Laplace_LongRange": (
+        "Line1d",
+        "System_Cuspy_LongRange",
+        dict(k_interactions=1.0, alpha=1.0, **_COMMON),
+    ),
+    "Line1d_Cuspy_Laplace_Nopassing": ("Line1d", "System_Cuspy_Laplace_Nopassing", _nopassing()),
+    "Line2d_Cuspy_Laplace": ("Line2d", "System_Cuspy_Laplace", _line2d()),
+}
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def example_system(module, name):
+    sub, cls, kw = EXAMPLES[name]
+    return getattr(getattr(module, sub), cls)(**kw)
+
+
+def quasistatic(system, nstep, eps=1e-3):
+    """examples/Line1d_Cuspy_Laplace.py:39-61: returns u_frame, mean f_frame,
+    S (sum of well-index changes) and the increment after each step."""
+    u_frame = np.empty(nstep)
+    f_frame = np.empty(nstep)
+    S = np.empty(nstep, dtype=np.int64)
+    inc = np.empty(nstep, dtype=np.int64)
+    for step in range(nstep):
+        i_n = np.copy(system.chunk.index_at_align)
+        if step == 0:
+            system.u_frame = 0.0
+        else:
+            system.eventDrivenStep(eps, step % 2 == 0)
+        if step % 2 == 0:
+            ret = system.minimise()
+            assert ret == 0
+        u_frame[step] = system.u_frame
+        f_frame[step] = np.mean(system.f_frame)
+        S[step] = np.sum(system.chunk.index_at_align - i_n)
+        inc[step] = system.inc
+    return u_frame, f_frame, S, inc
+
+
+def delta_par(N, **extra):
+    """The reference tests' delta landscape: yield positions at half integers
+    (test_Line1d.py:37-50)."""
+    par = dict(
+        m=1.0,
+        eta=1.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=0.1,
+        dt=1.0,
+        shape=[N],
+        seed=0,
+        distribution="delta",
+        parameters=[1.0],
+        offset=-49.5,
+        nchunk=100,
+    )
+    par.update(extra)
+    return par

@@ -1,0 +1,332 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Per particle the device evaluates the reference's expressions in the same
+order without FMA, so u, v, a, every force component and the landscape are
+compared bit for bit (np.array_equal). The StopList decision uses norms reduced
+in a different (fixed) order on the device; a decision could differ only if a
+residual fell within rounding of tol, so increments are compared exactly too.
+"""
+
+import numpy as np
+import pytest
+
+from tests import protocols as P
+from tests import reference_cases as RC
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["u", "v", "a", "f", "f_potential", "f_interactions", "f_frame", "f_damping"]
+
+
+@pytest.fixture(scope="module")
+def fq():
+    import frictionqpotspringblock_amd as fq
+
+    return fq
+
+
+def assert_same_state(a, b, what=""):
+    for name in FIELDS:
+        x, y = getattr(a, name), getattr(b, name)
+        assert np.array_equal(x, y), f"{what}: {name} differs, max |d| = {np.max(np.abs(x - y))}"
+    assert np.array_equal(a.chunk.index_at_align, b.chunk.index_at_align), what
+    assert a.inc == b.inc, (what, a.inc, b.inc)
+    assert a.u_frame == b.u_frame, what
+
+
+def _base(N, **kw):
+    par = dict(
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=3,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=60,
+    )
+    par.update(kw)
+    return par
+
+
+# (submodule, class, size, extra constructor arguments)
+SYSTEMS = [
+    ("Line1d", "System_Cuspy_Laplace", 1000, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 64, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 1537, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 4096, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 6001, dict(k_interactions=1.0)),
+    ("Line1d", "System_SemiSmooth_Laplace", 700, dict(k_interactions=1.0, kappa=1.0)),
+    ("Line1d", "System_Smooth_Laplace", 700, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Quartic", 900, dict(a1=1.0, a2=5.0)),
+    ("Line1d", "System_Cuspy_QuarticGradient", 900, dict(k2=1.0, k4=2.0)),
+    ("Line1d", "System_Cuspy_LongRange", 300, dict(k_interactions=1.0, alpha=1.5)),
+    ("Line2d", "System_Cuspy_Laplace", (30, 40), dict(k_interactions=1.0)),
+    ("Line2d", "System_Cuspy_QuarticGradient", (20, 25), dict(k2=1.0, k4=1.0)),
+]
+
+
+def _make(mod, sub, cls, n, extra, **kw):
+    shape = list(n) if isinstance(n, tuple) else [n]
+    size = int(np.prod(shape))
+    par = _base(size, shape=shape, **extra)
+    par.update(kw)
+    return getattr(getattr(mod, sub), cls)(**par)
+
+
+@pytest.mark.parametrize("sub,cls,n,extra", SYSTEMS, ids=[f"{s[1]}-{s[0]}-{s[2]}" for s in SYSTEMS])
+def test_dynamics_and_quasistatics(fq, oracle, sub, cls, n, extra):
+    a = _make(fq, sub, cls, n, extra)
+    b = _make(oracle, sub, cls, n, extra)
+    assert_same_state(a, b, "construction")
+    assert np.array_equal(a.chunk.data, b.chunk.data)
+    assert np.array_equal(a.chunk.start, b.chunk.start)
+    # large initial velocities: particles cross tens of wells in both
+    # directions, which exercises the chunk refills
+    v0 = np.random.default_rng(7).normal(0.0, 10.0, a.shape)
+    for s in (a, b):
+        s.v = v0
+    assert_same_state(a, b, "set_v")
+    for s in (a, b):
+        s.flowSteps(300, 0.2)
+    assert_same_state(a, b, "flowSteps")
+    assert np.array_equal(a.chunk.data, b.chunk.data), "landscape refill differs"
+    for s in (a, b):
+        s.timeSteps(7)
+    assert_same_state(a, b, "timeSteps")
+    ra = a.minimise()
+    rb = b.minimise()
+    assert ra == rb == 0
+    assert_same_state(a, b, "minimise")
+    if cls != "System_Smooth_Laplace":
+        for kick in (False, True):
+            da = a.eventDrivenStep(1e-3, kick)
+            db = b.eventDrivenStep(1e-3, kick)
+            assert da == db
+            assert_same_state(a, b, f"eventDrivenStep kick={kick}")
+        assert a.minimise(time_activity=True) == b.minimise(time_activity=True) == 0
+        assert a.quasistaticActivityFirst == b.quasistaticActivityFirst
+        assert a.quasistaticActivityLast == b.quasistaticActivityLast
+        assert_same_state(a, b, "minimise(time_activity)")
+    assert np.array_equal(a.chunk.data, b.chunk.data)
+
+
+def test_until_event_and_truncate(fq, oracle):
+    N = 500
+    a = _make(fq, "Line1d", "System_Cuspy_Laplace", N, dict(k_interactions=1.0))
+    b = _make(oracle, "Line1d", "System_Cuspy_Laplace", N, dict(k_interactions=1.0))
+    for s in (a, b):
+        s.minimise()
+        s.eventDrivenStep(1e-3, False)
+        s.eventDrivenStep(1e-3, True)
+    assert a.timeStepsUntilEvent() == b.timeStepsUntilEvent()
+    assert_same_state(a, b, "timeStepsUntilEvent")
+    i_n = a.chunk.index_at_align.copy()
+    assert a.minimise_truncate(i_n=i_n, A_truncate=3) == b.minimise_truncate(i_n=i_n, A_truncate=3)
+    assert_same_state(a, b, "minimise_truncate A")
+    assert a.minimise_truncate(i_n=i_n, S_truncate=20) == b.minimise_truncate(i_n=i_n, S_truncate=20)
+    assert_same_state(a, b, "minimise_truncate S")
+    assert a.quasistaticActivityFirst == b.quasistaticActivityFirst
+    assert a.quasistaticActivityLast == b.quasistaticActivityLast
+    for s in (a, b):
+        s.advanceToFixedForce(0.05, allow_plastic=True)
+    assert_same_state(a, b, "advanceToFixedForce")
+    for s in (a, b):
+        s.trigger(7, 1e-3)
+    assert np.array_equal(a.u, b.u)
+
+
+def test_nopassing(fq, oracle):
+    par = _base(800, k_interactions=1.0)
+    for k in ("m", "eta", "dt"):
+        par.pop(k)
+    a = fq.Line1d.System_Cuspy_Laplace_Nopassing(**par)
+    b = oracle.Line1d.System_Cuspy_Laplace_Nopassing(**par)
+    for step in range(6):
+        for s in (a, b):
+            if step:
+                s.eventDrivenStep(1e-3, step % 2 == 0)
+            assert s.minimise() == 0
+        assert_same_state(a, b, f"no-passing step {step}")
+
+
+@pytest.mark.parametrize("cls", ["System_Cuspy_Laplace_RandomForcing", "System_Cuspy_Quartic_RandomForcing"])
+def test_random_forcing(fq, oracle, cls):
+    N = 700
+    extra = dict(k_interactions=1.0) if "Laplace" in cls else dict(a1=1.0, a2=5.0)
+    rng = np.random.default_rng(0)
+    kw = dict(
+        mean=0.01,
+        stddev=0.1,
+        seed_forcing=5,
+        dinc_init=rng.integers(0, 12, N),
+        dinc=rng.integers(1, 12, N),
+        **extra,
+    )
+    a = _make(fq, "Line1d", cls, N, kw)
+    b = _make(oracle, "Line1d", cls, N, kw)
+    assert_same_state(a, b, "construction")
+    for s in (a, b):
+        s.flowSteps(250, 0.1)
+    assert_same_state(a, b, "flowSteps with random forcing")
+    assert np.array_equal(a.external.f_thermal, b.external.f_thermal)
+    assert np.array_equal(a.external.next, b.external.next)
+    assert a.external.state == b.external.state
+    for s in (a, b):
+        s.inc = s.inc + 5
+    assert_same_state(a, b, "set_inc")
+    assert not hasattr(a, "minimise")
+
+
+def test_reference_random_forcing(fq, oracle):
+    RC.random_forcing_draws(fq, oracle)
+
+
+@pytest.mark.parametrize("case", RC.ALL, ids=[f.__name__ for f in RC.ALL])
+def test_reference_case(fq, case):
+    case(fq)
+
+
+@pytest.mark.parametrize(
+    "name,nstep",
+    [
+        ("Line1d_Cuspy_Laplace", 200),
+        ("Line1d_Cuspy_Quartic", 120),
+        ("Line1d_SemiSmooth_Laplace", 200),
+        ("Line1d_Cuspy_Laplace_Nopassing", 400),
+        ("Line2d_Cuspy_Laplace", 60),
+        ("Line1d_Cuspy_Laplace_LongRange", 6),
+    ],
+)
+def test_golden_examples(fq, oracle, name, nstep):
+    ref = P.golden(name)
+    a = P.example_system(fq, name)
+    ua, fa, Sa, ia = P.quasistatic(a, nstep)
+    assert np.array_equal(Sa, ref["S"][:nstep])
+    assert np.allclose(ua, ref["x_frame"][:nstep])
+    assert np.allclose(fa, ref["f_frame"][:nstep])
+    b = P.example_system(oracle, name)
+    ub, fb, Sb, ib = P.quasistatic(b, nstep)
+    assert np.array_equal(ia, ib)
+    assert np.array_equal(ua, ub)
+    assert_same_state(a, b, name)
+
+
+def test_landscape_far_moves(fq, oracle):
+    for dist, par in [("weibull", [2.0, 1.1, 1e-3]), ("weibull", [3.0, 1.0]), ("random", [2.0]), ("exponential", [0.5])]:
+        p = P.delta_par(4, distribution=dist, parameters=par, offset=-30.0, nchunk=40, seed=9)
+        a = fq.Line1d.System_Cuspy_Laplace(**p)
+        b = oracle.Line1d.System_Cuspy_Laplace(**p)
+        assert np.array_equal(a.chunk.data, b.chunk.data), dist
+        for u in [np.array([3.0, 50.0, 400.0, -70.0]), np.array([-100.0, 20.0, 10.0, 0.5]), np.zeros(4)]:
+            a.u = u
+            b.u = u
+            assert np.array_equal(a.chunk.data, b.chunk.data), (dist, u)
+            assert np.array_equal(a.chunk.start, b.chunk.start)
+            assert np.array_equal(a.chunk.index_at_align, b.chunk.index_at_align)
+            assert np.array_equal(a.f_potential, b.f_potential)
+        st = a.chunk.start.copy()
+        assert np.array_equal(a.chunk.state_at(st), b.chunk.state_at(st))
+        a.chunk.restore(state=b.chunk.state_at(st), value=b.chunk.data[..., 0], index=st)
+        assert np.array_equal(a.chunk.data, b.chunk.data)
+
+
+def test_batch_equals_single(fq, oracle):
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R, N = 5, 1024
+    kw = dict(
+        potential="Cuspy",
+        interactions="Laplace1d",
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=0,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=200,
+    )
+    batch = Batch(R, seed_stride=1, **kw)
+    batch.flowSteps(2000, 1e-3, block=700)
+    single = []
+    for r in range(R):
+        par = _base(N, k_interactions=1.0, seed=r, nchunk=200)
+        s = fq.Line1d.System_Cuspy_Laplace(**par)
+        s.flowSteps(2000, 1e-3)
+        single.append(s)
+        assert np.array_equal(batch.u[r], s.u)
+        assert np.array_equal(batch.v[r], s.v)
+        assert np.array_equal(batch.f[r], s.f)
+        assert np.array_equal(batch.index_at_align[r], s.chunk.index_at_align)
+        assert batch.u_frame[r] == s.u_frame
+    o = oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, seed=2, nchunk=200))
+    o.flowSteps(2000, 1e-3)
+    assert np.array_equal(batch.u[2], o.u)
+    # batched minimise equals single minimise
+    assert np.all(batch.minimise() == 0)
+    for r in range(R):
+        assert single[r].minimise() == 0
+        assert np.array_equal(batch.u[r], single[r].u)
+        assert batch.inc[r] == single[r].inc
+    # batched event-driven step equals single
+    du = batch.eventDrivenStep(1e-3, False)
+    for r in range(R):
+        assert du[r] == single[r].eventDrivenStep(1e-3, False)
+        assert np.array_equal(batch.u[r], single[r].u)
+
+
+def test_launch_shapes_agree(fq):
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R, N = 3, 4096
+    kw = dict(
+        m=1.0,
+        eta=0.3,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=0,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        nchunk=200,
+    )
+    ref = None
+    for ppt, threads in [(4, 1024), (8, 512), (16, 256), (32, 128), (8, 1024)]:
+        b = Batch(R, **kw)
+        b.set_launch(ppt, threads)
+        assert b.launch == (ppt, threads)
+        b.flowSteps(500, 1e-2)
+        if ref is None:
+            ref = b.u
+        assert np.array_equal(b.u, ref), (ppt, threads)
+
+
+def test_errors(fq):
+    s = fq.Line1d.System_Cuspy_Laplace(**P.delta_par(5))
+    s.u = np.full(5, np.nan)
+    with pytest.raises(RuntimeError, match="NaN entries found"):
+        s.timeStep()
+    s = fq.Line1d.System_Cuspy_Laplace(**P.delta_par(5))
+    s.u = np.array([0.3, 0, 0, 0, 0])
+    with pytest.raises(RuntimeError, match="No convergence found"):
+        s.minimise(max_iter=1)
+    s = fq.Line1d.System_Cuspy_Laplace(**P.delta_par(5))
+    s.u = np.array([0.3, 0, 0, 0, 0])
+    assert s.minimise(max_iter=1, max_iter_is_error=False) == 2
+    smooth = fq.Line1d.System_Smooth_Laplace(**P.delta_par(5))
+    with pytest.raises(RuntimeError, match="Operation not possible."):
+        smooth.maxUniformDisplacement()
+    with pytest.raises(RuntimeError, match="Deprecated"):
+        s.x
+    with pytest.raises(RuntimeError, match="Deprecated"):
+        s.f_neighbours
