@@ -86,7 +86,10 @@ def test_dynamics_and_quasistatics(fq, oracle, sub, cls, n, extra):
     assert np.array_equal(a.chunk.start, b.chunk.start)
     # large initial velocities: particles cross tens of wells in both
     # directions, which exercises the chunk refills
-    v0 = np.random.default_rng(7).normal(0.0, 10.0, a.shape)
+    # (nonlinear interactions get a gentler kick: a cubic spring force at
+    # dt = 0.1 is unstable for neighbour differences of several wells)
+    scale = 1.0 if "Quartic" in cls else 10.0
+    v0 = np.random.default_rng(7).normal(0.0, scale, a.shape)
     for s in (a, b):
         s.v = v0
     assert_same_state(a, b, "set_v")
@@ -228,10 +231,15 @@ def test_landscape_far_moves(fq, oracle):
             assert np.array_equal(a.chunk.start, b.chunk.start)
             assert np.array_equal(a.chunk.index_at_align, b.chunk.index_at_align)
             assert np.array_equal(a.f_potential, b.f_potential)
-        st = a.chunk.start.copy()
-        assert np.array_equal(a.chunk.state_at(st), b.chunk.state_at(st))
-        a.chunk.restore(state=b.chunk.state_at(st), value=b.chunk.data[..., 0], index=st)
+        st = a.chunk.start.copy() - 7
+        state = b.chunk.state_at(st)
+        assert np.array_equal(a.chunk.state_at(st), state)
+        value = b.chunk.data[..., 0] - 1.0
+        for s in (a, b):
+            s.chunk.restore(state=state, value=value, index=st)
+            s.u = np.array([-40.0, 3.0, 30.0, 0.0])
         assert np.array_equal(a.chunk.data, b.chunk.data)
+        assert np.array_equal(a.chunk.index_at_align, b.chunk.index_at_align)
 
 
 def test_batch_equals_single(fq, oracle):
