@@ -278,7 +278,7 @@ class Batch:
         return p.value, t.value
 
     # launch shapes (particles per thread, threads per workgroup) the autotuner tries
-    CANDIDATES = [(1, 1024), (2, 1024), (4, 1024), (8, 512), (8, 1024), (16, 256), (16, 512), (32, 128), (32, 256)]
+    CANDIDATES = [(4, 1024), (4, 512), (8, 512), (8, 256), (16, 256), (16, 512), (8, 1024)]
 
     def autotune(self, v_frame=0.0, steps=500, candidates=None):
         """Time one resident block per launch shape and keep the fastest.

@@ -294,33 +294,40 @@ struct Group {
         return false;
     }
 
-    // Run `total` steps (or until every system stopped) in launches of `block`.
-    // Returns the accumulated time between events around the launches (ms).
-    float run(fq::RunArgs ra, uint64_t total, uint64_t block, bool check_each, bool timed)
+    // Run up to `total` steps per system (or until every system stopped) in
+    // launches of at most `block` steps. A launch that stops before a step
+    // needing a landscape refill (ST_REFILL) is followed by k_refill and a
+    // relaunch. Returns the time between events around all launches (ms).
+    float run(fq::RunArgs ra, uint64_t total, uint64_t block, bool timed)
     {
         reset_run(ra.mode);
+        for (auto& r : hrs) {
+            r.target = (int64_t)total;
+        }
         push();
         if (timed) {
             HIPC(hipEventRecord(ev0, stream));
         }
-        uint64_t done = 0;
-        while (done < total) {
-            const uint64_t nb = std::min(block, total - done);
-            ra.max_steps = (int64_t)nb;
+        ra.max_steps = (int64_t)std::min<uint64_t>(block, (uint64_t)std::numeric_limits<int32_t>::max());
+        while (total > 0) {
             launch_resident(ra);
-            done += nb;
-            if (check_each && done < total) {
-                pull();
-                if (!any_running()) {
-                    break;
+            pull();
+            bool refill = false, more = false;
+            for (const auto& r : hrs) {
+                refill |= r.status == fq::ST_REFILL;
+                more |= (r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) && r.steps < r.target;
+            }
+            if (refill) {
+                HIPC(fq::launch_refill(views, (int)R, stream));
+                for (auto& r : hrs) {
+                    if (r.status == fq::ST_REFILL) {
+                        r.status = fq::ST_RUNNING;
+                    }
                 }
-                bool nan = false;
-                for (const auto& r : hrs) {
-                    nan |= r.status == fq::ST_NAN;
-                }
-                if (nan) {
-                    break;
-                }
+                continue;
+            }
+            if (!more) {
+                break;
             }
         }
         float ms = 0.0f;
@@ -339,7 +346,8 @@ struct Group {
         ra.mode = fq::MODE_STEPS;
         ra.flow = flow ? 1 : 0;
         ra.v_frame = v_frame;
-        float ms = run(ra, n, block ? block : STEP_BLOCK, check_each, timed);
+        (void)check_each;
+        float ms = run(ra, n, block ? block : STEP_BLOCK, timed);
         for (const auto& r : hrs) {
             if (r.status == fq::ST_NAN) {
                 throw Error("NaN entries found");
@@ -848,7 +856,7 @@ int fqsb_time_steps_until_event(fqsb_system* s, double tol, uint64_t niter_tol, 
         ra.tol = tol;
         ra.tol2 = tol * tol;
         ra.niter_tol = (int64_t)niter_tol;
-        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        g.run(ra, max_iter, MIN_BLOCK, false);
         const auto& r = g.hrs[0];
         if (r.status == fq::ST_NAN) {
             throw Error("NaN entries found");
@@ -932,7 +940,7 @@ int fqsb_minimise(fqsb_system* s, double tol, uint64_t niter_tol, uint64_t max_i
         ra.tol = tol;
         ra.tol2 = tol * tol;
         ra.niter_tol = (int64_t)niter_tol;
-        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        g.run(ra, max_iter, MIN_BLOCK, false);
         const auto& r = g.hrs[0];
         if (r.status == fq::ST_NAN) {
             throw Error("NaN entries found");
@@ -966,7 +974,7 @@ int fqsb_minimise_truncate(fqsb_system* s, const int64_t* i_n, uint64_t A_trunca
         ra.niter_tol = (int64_t)niter_tol;
         ra.A_truncate = (int64_t)A_truncate;
         ra.S_truncate = (int64_t)S_truncate;
-        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        g.run(ra, max_iter, MIN_BLOCK, false);
         const auto& r = g.hrs[0];
         if (r.status == fq::ST_NAN) {
             throw Error("NaN entries found");
@@ -1231,7 +1239,7 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
         ra.tol = tol;
         ra.tol2 = tol * tol;
         ra.niter_tol = (int64_t)niter_tol;
-        g.run(ra, max_iter, MIN_BLOCK, true, false);
+        g.run(ra, max_iter, MIN_BLOCK, false);
         for (int64_t r = 0; r < g.R; ++r) {
             const auto& st = g.hrs[(size_t)r];
             if (st.status == fq::ST_NAN) {

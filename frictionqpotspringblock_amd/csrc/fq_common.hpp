@@ -54,7 +54,8 @@ enum Status : int32_t {
     ST_EVENT = 2,     // a well index changed (timeStepsUntilEvent)
     ST_TRUNCATED = 3, // A_truncate / S_truncate reached
     ST_NAN = 4,       // NaN in u
-    ST_BUDGET = 5     // step budget of this launch used up
+    ST_BUDGET = 5,    // step budget of this launch used up
+    ST_REFILL = 6     // the next step needs a landscape refill (k_refill, then relaunch)
 };
 
 // Components recomputed by k_forces (fq_service.hpp).
@@ -91,7 +92,8 @@ struct Landscape {
 struct RunState {
     int64_t inc;
     double u_frame;
-    int64_t steps; // steps completed by the current run (all launches)
+    int64_t steps;  // steps completed by the current run (all launches)
+    int64_t target; // steps the current run may take in total
     int32_t status;
     int32_t act_init; // time_activity: still waiting for the first event
     // StopList (GooseFEM Iterate::StopList) as counters, see fq_resident.hpp

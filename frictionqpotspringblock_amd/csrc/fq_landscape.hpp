@@ -74,7 +74,7 @@ __device__ __noinline__ void chunk_shift_to(const Landscape& L, int64_t p, int64
 // Slow path of the alignment of particle p to slip u (u outside its current
 // well, or the well too close to the window edge). Returns the new local
 // index; afterwards y[j] <= u < y[j+1] and buffer <= j <= n - 2 - buffer.
-__device__ __noinline__ int32_t chunk_align_slow(const Landscape& L, int64_t p, double u, int32_t j0)
+__device__ __forceinline__ int32_t chunk_align_slow_inl(const Landscape& L, int64_t p, double u, int32_t j0)
 {
     const int64_t n = L.nchunk;
     const double* y = L.y + p * n;
@@ -122,6 +122,11 @@ __device__ __noinline__ int32_t chunk_align_slow(const Landscape& L, int64_t p, 
     }
     L.jloc[p] = (int32_t)j;
     return (int32_t)j;
+}
+
+__device__ __noinline__ int32_t chunk_align_slow(const Landscape& L, int64_t p, double u, int32_t j0)
+{
+    return chunk_align_slow_inl(L, p, u, j0);
 }
 
 // Align particle p (registers yl, yr, j hold its current well). Returns the

@@ -46,6 +46,26 @@ constexpr int res_max_threads()
     return PPT <= 8 ? 1024 : 512;
 }
 
+// Values that are equal in every lane (read back from LDS after a barrier),
+// made visibly uniform so that loop control and counters stay scalar.
+__device__ __forceinline__ int uniform_i(int x)
+{
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
+__device__ __forceinline__ long long uniform_ll(long long x)
+{
+    const uint64_t b = (uint64_t)x;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return (long long)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double uniform_d(double x)
+{
+    return __builtin_bit_cast(double, uniform_ll(__builtin_bit_cast(long long, x)));
+}
+
 __device__ __forceinline__ double wave_sum(double x)
 {
     for (int off = 32; off > 0; off >>= 1) {
@@ -123,6 +143,24 @@ __device__ __forceinline__ long long block_min_i(long long x, long long* s_w)
     return r;
 }
 
+// Is the well of slip x reachable from well j (yl, yr) by walking inside the
+// safe zone [jmin, jmax] of the window `row`? Reads only.
+__device__ __forceinline__ bool reachable(const double* row, int32_t j, double lo, double hi, double x, int32_t jmin,
+                                          int32_t jmax)
+{
+    while (x >= hi && j < jmax) {
+        ++j;
+        lo = hi;
+        hi = row[j + 1];
+    }
+    while (x < lo && j > jmin) {
+        --j;
+        hi = lo;
+        lo = row[j];
+    }
+    return lo <= x && x < hi;
+}
+
 // LDS needed by a resident launch (bytes of dynamic shared memory).
 template <int INT>
 FQ_HD size_t resident_lds_bytes(int64_t N, int threads)
@@ -142,15 +180,19 @@ template <int POT, int INT, int EXT, int PPT, int MAXT>
 __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ views, RunArgs ra)
 {
     constexpr bool CHAIN = is_chain<INT>();
-    const SysView S = views[blockIdx.x];
-    RunState* rs = S.rs;
+    const SysView& S = views[blockIdx.x];
+    RunState* const rs = S.rs;
     if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
         return;
     }
+    // steps this launch may take: the launch budget, capped by the run's target
+    const int64_t room = rs->target - rs->steps;
+    const int limit = (int)(room < ra.max_steps ? (room > 0 ? room : 0) : ra.max_steps);
 
     __shared__ double s_red_d[2][RES_MAX_WAVES][2];
     __shared__ long long s_red_i[2][RES_MAX_WAVES][2];
     __shared__ int s_nan[2];
+    __shared__ int s_refill[2];
     __shared__ int s_scan_w[RES_MAX_WAVES];
     __shared__ long long s_min_w[RES_MAX_WAVES];
     extern __shared__ double s_dyn[];
@@ -161,18 +203,22 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
     const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
     const int Tact = (int)((N + PPT - 1) / PPT);
     const Landscape& L = S.ls;
-    const bool need_delta = ra.mode == MODE_UNTIL_EVENT || ra.mode == MODE_TRUNCATE ||
-                            (ra.mode == MODE_MINIMISE && ra.time_activity);
+    const bool track = ra.mode != MODE_STEPS;
+
+    // hot scalars
+    double* const Y = L.y;
+    const int64_t nch = L.nchunk;
+    const int32_t jmin = (int32_t)L.buffer;
+    const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
 
     // LDS exchange buffers
-    double* h_first = s_dyn;         // [2][T]   chain: first slip of each thread
-    double* h_last = s_dyn + 2 * T;  // [2][T]   chain: last slip of each thread
-    double* mir = s_dyn;             // [2][N]   lattice / long range: all slips
-    double* pref = s_dyn + 2 * N;    // [N]      long range prefactor
+    double* h_first = s_dyn;        // [2][T]   chain: first slip of each thread
+    double* h_last = s_dyn + 2 * T; // [2][T]   chain: last slip of each thread
+    double* mir = s_dyn;            // [2][N]   lattice / long range: all slips
+    double* pref = s_dyn + 2 * N;   // [N]      long range prefactor
 
-    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], fth[PPT], uprev[PPT];
-    int32_t jl[PPT];
-    int64_t dl[PPT];
+    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], fth[PPT];
+    int32_t jl[PPT], dl[PPT];
 
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
@@ -185,25 +231,41 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
             v[i] = S.v[p];
             a[i] = S.a[p];
             jl[i] = L.jloc[p];
-            yl[i] = L.y[p * L.nchunk + jl[i]];
-            yr[i] = L.y[p * L.nchunk + jl[i] + 1];
+            yl[i] = Y[p * nch + jl[i]];
+            yr[i] = Y[p * nch + jl[i] + 1];
             if constexpr (EXT) {
                 fth[i] = S.f_th[p];
             }
-            if (need_delta) {
-                dl[i] = L.start[p] + jl[i] - S.i_ref[p];
+            if (track) {
+                dl[i] = (int32_t)(L.start[p] + jl[i] - S.i_ref[p]);
             }
         }
     }
 
-    auto publish = [&](int b) {
+    const double dt = S.dt;
+    const double c2 = 0.5 * dt * dt;
+    const double hdt = 0.5 * dt;
+    const double meta = -S.eta;
+    const double inv_m = S.inv_m;
+    const double kf = S.k_frame;
+    const double dv = ra.v_frame * dt;
+
+    // Publish the slips that other threads read: the current ones (`next`
+    // false) or those after the position update of the coming step, computed
+    // without touching the registers.
+    auto publish = [&](int b, bool next) {
         if constexpr (CHAIN) {
             if (cnt > 0) {
-                h_first[b * T + t] = u[0];
 #pragma unroll
                 for (int i = 0; i < PPT; ++i) {
-                    if (i == cnt - 1) {
-                        h_last[b * T + t] = u[i];
+                    if (i == 0 || i == cnt - 1) {
+                        const double x = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
+                        if (i == 0) {
+                            h_first[b * T + t] = x;
+                        }
+                        if (i == cnt - 1) {
+                            h_last[b * T + t] = x;
+                        }
                     }
                 }
             }
@@ -212,7 +274,7 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 if (i < cnt) {
-                    mir[(int64_t)b * N + base + i] = u[i];
+                    mir[(int64_t)b * N + base + i] = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
                 }
             }
         }
@@ -223,10 +285,12 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
             pref[q] = S.pref[q];
         }
     }
-    publish(0);
+    publish(0, false);
     if (t == 0) {
         s_nan[0] = 0;
         s_nan[1] = 0;
+        s_refill[0] = 0;
+        s_refill[1] = 0;
     }
 
     int64_t inc = rs->inc;
@@ -240,14 +304,6 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
     const uint64_t rng_inc = rs->rng_inc;
     int64_t min_next = rs->min_next;
 
-    const double dt = S.dt;
-    const double c2 = 0.5 * dt * dt;
-    const double hdt = 0.5 * dt;
-    const double meta = -S.eta;
-    const double inv_m = S.inv_m;
-    const double kf = S.k_frame;
-    const double dv = ra.v_frame * dt;
-
     if constexpr (EXT) {
         long long mn = 0x7fffffffffffffffLL;
 #pragma unroll
@@ -257,15 +313,14 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                 mn = nx < mn ? nx : mn;
             }
         }
-        min_next = block_min_i(mn, s_min_w);
+        min_next = uniform_ll(block_min_i(mn, s_min_w));
     }
     __syncthreads();
 
-    // neighbour access of a chain particle
     auto left_of = [&](int b) -> double { return h_last[b * T + (t == 0 ? Tact - 1 : t - 1)]; };
     auto right_of = [&](int b) -> double { return h_first[b * T + (t == Tact - 1 ? 0 : t + 1)]; };
 
-    // interaction force of local particle i from exchange buffer b
+    // interaction force of local particle i, neighbours from exchange buffer b
     auto interaction = [&](int i, int b) -> double {
         if constexpr (INT == INT_NONE) {
             return 0.0;
@@ -275,13 +330,8 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 if (q == i) {
-                    um = (q == 0) ? left_of(b) : u[q - 1];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                if (q == i) {
-                    up = (q == cnt - 1) ? right_of(b) : (q + 1 < PPT ? u[q + 1] : 0.0);
+                    um = (q == 0) ? left_of(b) : u[q > 0 ? q - 1 : 0];
+                    up = (q == cnt - 1) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
                 }
             }
             return chain_force<INT>(S, um, u[i], up);
@@ -302,34 +352,47 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
     int it;
     for (it = 1;; ++it) {
         const int b = it & 1;
-        // A: speculative position update of step `it`
-        int nanf = 0;
+        // A: publish the slips after the position update of step `it`, and
+        // check (read only) that every particle's new well is reachable inside
+        // its window; otherwise step `it` is left to a refill and a relaunch
+        publish(b, true);
+        {
+            int refill = 0;
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            uprev[i] = u[i];
-            if (i < cnt) {
-                u[i] = u[i] + dt * v[i] + c2 * a[i];
-                nanf |= __builtin_isnan(u[i]);
+            for (int i = 0; i < PPT; ++i) {
+                if (i < cnt) {
+                    const double x = u[i] + dt * v[i] + c2 * a[i];
+                    if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
+                        refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
+                    }
+                }
             }
-        }
-        publish(b);
-        if (nanf) {
-            s_nan[b] = it;
+            if (refill) {
+                s_refill[b] = it;
+            }
         }
         __syncthreads();
 
-        // B: stop decision for step it-1
-        if (ra.mode != MODE_STEPS && it > 1) {
+        // B: decisions on step it-1
+        if (it > 1 && uniform_i(s_nan[(it - 1) & 1]) == it - 1) {
+            status = ST_NAN;
+            break;
+        }
+        if (track && it > 1) {
             const int pb = (it - 1) & 1;
             double sf = 0.0, sff = 0.0;
-            long long nch = 0, sabs = 0;
+            long long nchg = 0, sabs = 0;
             for (int w = 0; w < nw; ++w) {
                 sf += s_red_d[pb][w][0];
                 sff += s_red_d[pb][w][1];
-                nch += s_red_i[pb][w][0];
+                nchg += s_red_i[pb][w][0];
                 sabs += s_red_i[pb][w][1];
             }
-            if (ra.mode == MODE_UNTIL_EVENT && nch > 0) {
+            sf = uniform_d(sf);
+            sff = uniform_d(sff);
+            nchg = uniform_ll(nchg);
+            sabs = uniform_ll(sabs);
+            if (ra.mode == MODE_UNTIL_EVENT && nchg > 0) {
                 status = ST_EVENT;
             }
             else {
@@ -355,7 +418,7 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                     status = ST_CONVERGED;
                 }
                 else if (ra.mode == MODE_TRUNCATE) {
-                    if (ra.A_truncate > 0 && nch >= ra.A_truncate) {
+                    if (ra.A_truncate > 0 && nchg >= ra.A_truncate) {
                         status = ST_TRUNCATED;
                     }
                     else if (ra.S_truncate > 0 && sabs >= ra.S_truncate) {
@@ -364,20 +427,15 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                 }
             }
         }
-        if (status != ST_RUNNING || it > ra.max_steps) {
-#pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                u[i] = uprev[i];
-            }
+        if (status != ST_RUNNING || it > limit) {
             break;
         }
-        if (s_nan[b] == it) {
-            status = ST_NAN;
-            inc += 1;
+        if (uniform_i(s_refill[b]) == it) {
+            status = ST_REFILL;
             break;
         }
 
-        // C: complete step `it`
+        // C: step `it`
         inc += 1;
         if (ra.flow) {
             uf = uf + dv;
@@ -393,7 +451,6 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                 }
                 int total = 0;
                 int rank = block_excl_scan(due, s_scan_w, total);
-                long long mn = 0x7fffffffffffffffLL;
                 if (due > 0) {
                     uint64_t st = pcg_advance(rng, rng_inc, 2 * (int64_t)rank);
 #pragma unroll
@@ -403,12 +460,12 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                             int64_t nx = S.next[p];
                             if (inc >= nx) {
                                 fth[i] = draw_normal(st, rng_inc, S.mean, S.stddev);
-                                nx += S.dinc[p];
-                                S.next[p] = nx;
+                                S.next[p] = nx + S.dinc[p];
                             }
                         }
                     }
                 }
+                long long mn = 0x7fffffffffffffffLL;
 #pragma unroll
                 for (int i = 0; i < PPT; ++i) {
                     if (i < cnt) {
@@ -416,18 +473,52 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                         mn = nx < mn ? nx : mn;
                     }
                 }
-                rng = pcg_advance(rng, rng_inc, 2 * (int64_t)total);
-                min_next = block_min_i(mn, s_min_w);
+                rng = pcg_advance(rng, rng_inc, 2 * (int64_t)uniform_i(total));
+                min_next = uniform_ll(block_min_i(mn, s_min_w));
             }
         }
 
-        double sf = 0.0, sff = 0.0;
-        long long nch = 0, sabs = 0;
+        // position update and well search inside the window (phase A made
+        // sure that the walk stays inside it; a non-finite slip keeps its well)
+        int nanf = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             if (i < cnt) {
-                const int64_t p = base + i;
-                dl[i] += align_particle(L, p, u[i], yl[i], yr[i], jl[i]);
+                u[i] = u[i] + dt * v[i] + c2 * a[i];
+                const double x = u[i];
+                nanf |= __builtin_isnan(x);
+                if (!(yl[i] <= x && x < yr[i])) {
+                    const double* row = Y + (base + i) * nch;
+                    int32_t j = jl[i];
+                    double lo = yl[i], hi = yr[i];
+                    while (x >= hi && j < jmax) {
+                        ++j;
+                        lo = hi;
+                        hi = row[j + 1];
+                    }
+                    while (x < lo && j > jmin) {
+                        --j;
+                        hi = lo;
+                        lo = row[j];
+                    }
+                    if (lo <= x && x < hi) {
+                        dl[i] += j - jl[i];
+                        jl[i] = j;
+                        yl[i] = lo;
+                        yr[i] = hi;
+                    }
+                }
+            }
+        }
+        if (nanf) {
+            s_nan[b] = it;
+        }
+
+        double sf = 0.0, sff = 0.0;
+        int nchg = 0, sabs = 0;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
                 const double fi = interaction(i, b);
                 const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
                 const double ff = kf * (uf - u[i]);
@@ -444,22 +535,23 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
                 aa = f * inv_m;
                 v[i] = vv;
                 a[i] = aa;
-                sf += f * f;
-                sff += ff * ff;
-                nch += dl[i] != 0;
-                sabs += dl[i] < 0 ? -dl[i] : dl[i];
+                if (track) {
+                    sf += f * f;
+                    sff += ff * ff;
+                    nchg += dl[i] != 0;
+                    sabs += dl[i] < 0 ? -dl[i] : dl[i];
+                }
             }
         }
-        if (ra.mode != MODE_STEPS) {
+        if (track) {
             sf = wave_sum(sf);
             sff = wave_sum(sff);
-            nch = wave_sum_i(nch);
-            sabs = wave_sum_i(sabs);
+            long long n1 = wave_sum_i(nchg), n2 = wave_sum_i(sabs);
             if (lane == 0) {
                 s_red_d[b][wave][0] = sf;
                 s_red_d[b][wave][1] = sff;
-                s_red_i[b][wave][0] = nch;
-                s_red_i[b][wave][1] = sabs;
+                s_red_i[b][wave][0] = n1;
+                s_red_i[b][wave][1] = n2;
             }
         }
     }
@@ -476,7 +568,9 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
         status = ST_BUDGET;
     }
 
-    // write back the state and the forces of the final configuration
+    // write back the state and the forces of the final configuration; the
+    // exchange buffer done&1 holds its slips (published at the start of the
+    // iteration after the last completed step, or initially)
     const int fb = done & 1;
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {

@@ -26,6 +26,12 @@ hipError_t launch_trigger(const SysView* views, int64_t p, double eps, int32_t d
     return hipGetLastError();
 }
 
+hipError_t launch_refill(const SysView* views, int nsys, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_refill, dim3(nsys), dim3(SVC_THREADS), 0, stream, views);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_iref(const SysView* views, int nsys, int64_t maxN, hipStream_t stream)
 {
     dim3 grid((unsigned)((maxN + SVC_THREADS - 1) / SVC_THREADS), (unsigned)nsys);

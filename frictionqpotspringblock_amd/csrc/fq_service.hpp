@@ -251,6 +251,34 @@ __global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double e
     S.u[p] = direction > 0 ? yr + 0.5 * eps : yl - 0.5 * eps;
 }
 
+// Landscape refill requested by a resident launch (status ST_REFILL): align
+// every particle to its slip after the position update of the pending step,
+// u + dt v + dt^2/2 a (bit-identical to the resident kernel), shifting and
+// regenerating its window like the CPU align; then the run continues.
+__global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restrict__ views)
+{
+    const SysView& S = views[blockIdx.x];
+    RunState* rs = S.rs;
+    if (rs->status != ST_REFILL) {
+        return;
+    }
+    const Landscape& L = S.ls;
+    const double dt = S.dt;
+    const double c2 = 0.5 * dt * dt;
+    for (int64_t p = threadIdx.x; p < S.N; p += blockDim.x) {
+        const double x = S.u[p] + dt * S.v[p] + c2 * S.a[p];
+        const int32_t j = L.jloc[p];
+        const double* row = L.y + p * L.nchunk;
+        if (!(row[j] <= x && x < row[j + 1])) {
+            chunk_align_slow(L, p, x, j);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rs->status = ST_RUNNING;
+    }
+}
+
 // Reference index for event / activity counting: i_ref = start + jloc.
 __global__ void __launch_bounds__(SVC_THREADS) k_set_iref(const SysView* __restrict__ views)
 {
