@@ -50,24 +50,32 @@ PARAMS = dict(
 )
 
 
-def cpu_baseline(replicas=64, steps=500):
+def cpu_baseline(replicas=256, target_s=10.0):
     """The oracle on the host: `replicas` systems of the same workload, one per
-    thread, `steps` flowSteps each."""
+    thread, flowSteps sized from a short pilot to take about `target_s`."""
     from oracle import pyoracle
 
     pyoracle.lib()
     cores = min(16, os.cpu_count() or 1)
     systems = [pyoracle.Line1d.System_Cuspy_Laplace(**dict(PARAMS, seed=r)) for r in range(replicas)]
-    t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
-        list(ex.map(lambda s: s.flowSteps(steps, V_FRAME), systems))
-    dt = time.perf_counter() - t0
+
+    def sweep(n):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+            list(ex.map(lambda s: s.flowSteps(n, V_FRAME), systems))
+        return time.perf_counter() - t0
+
+    pilot = 200
+    dt_pilot = sweep(pilot)
+    steps = max(pilot, int(pilot * target_s / max(dt_pilot, 1e-3)))
+    dt = sweep(steps)
     return {
         "value": replicas * N * steps / dt,
         "unit": "particle-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{replicas} replicas x N={N} x {steps} flowSteps, one replica per thread ({dt:.1f} s)",
+        "sample": f"{replicas} replicas x N={N} x {steps} flowSteps of the same workload, "
+        f"one replica per thread ({dt:.1f} s)",
     }
 
 

@@ -280,13 +280,16 @@ class Batch:
     # launch shapes (particles per thread, threads per workgroup) the autotuner tries
     CANDIDATES = [(4, 1024), (4, 512), (8, 512), (8, 256), (16, 256), (16, 512), (8, 1024)]
 
-    def autotune(self, v_frame=0.0, steps=500, candidates=None):
-        """Time one resident block per launch shape and keep the fastest.
+    def autotune(self, v_frame=0.0, steps=500, candidates=None, repeats=2):
+        """Time resident blocks per launch shape and keep the fastest.
 
-        The trial steps advance the batch like flowSteps(steps, v_frame) per
-        candidate; use it during warm-up. Returns {(ppt, threads): ms}.
+        One untimed block first (clock ramp), then `repeats` timed blocks per
+        candidate, best time kept. The trial steps advance the batch like
+        flowSteps(steps, v_frame); use it during warm-up. Returns
+        {(ppt, threads): ms}.
         """
         timings = {}
+        self.flowSteps(steps, v_frame, block=steps)
         for ppt, threads in candidates or self.CANDIDATES:
             need = -(-self._N // ppt)  # threads that own particles
             if need > threads or need <= threads // 2:
@@ -295,7 +298,9 @@ class Batch:
                 self.set_launch(ppt, threads)
             except RuntimeError:
                 continue
-            timings[(ppt, threads)] = self.flowSteps(steps, v_frame, block=steps, timed=True)
+            timings[(ppt, threads)] = min(
+                self.flowSteps(steps, v_frame, block=steps, timed=True) for _ in range(repeats)
+            )
         if not timings:
             self.set_launch(0, 0)
             return timings
