@@ -41,7 +41,14 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-HIP_SOURCES = ["fq_capi.hip", "fq_service.hip", "fq_res_cuspy.hip", "fq_res_semismooth.hip", "fq_res_smooth.hip"]
+HIP_SOURCES = [
+    "fq_capi.hip",
+    "fq_service.hip",
+    "fq_stream.hip",
+    "fq_res_cuspy.hip",
+    "fq_res_semismooth.hip",
+    "fq_res_smooth.hip",
+]
 
 
 def _run(cmd):

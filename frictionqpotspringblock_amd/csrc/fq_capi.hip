@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -92,7 +93,9 @@ struct Group {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int ppt = 0, maxt = 0, threads = 0;
-    bool user_launch = false;
+    bool streaming = false; // systems too large for a resident workgroup
+    double *u2 = nullptr, *v2 = nullptr, *a2 = nullptr, *yl = nullptr, *yr = nullptr, *yl2 = nullptr, *yr2 = nullptr;
+    int32_t* jl2 = nullptr;
     size_t lds_limit = 65536;
 
     Group() = default;
@@ -124,6 +127,14 @@ struct Group {
         dfree(st_end);
         dfree(rs);
         dfree(views);
+        dfree(u2);
+        dfree(v2);
+        dfree(a2);
+        dfree(yl);
+        dfree(yr);
+        dfree(yl2);
+        dfree(yr2);
+        dfree(jl2);
         if (ev0) {
             (void)hipEventDestroy(ev0);
         }
@@ -260,6 +271,13 @@ struct Group {
     {
         if (minim == FQSB_MIN_OVERDAMPED) {
             throw Error("dynamics are not available for no-passing systems");
+        }
+        if (streaming) {
+            if (!fq::launch_stream(pot, inter, ext, views, (int)R, ra, stream)) {
+                throw Error("no streaming kernel for this potential / interaction / external combination");
+            }
+            HIPC(hipGetLastError());
+            return;
         }
         size_t lds = fq::resident_lds(inter, N, threads);
         if (!fq::launch_resident(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream)) {
@@ -573,6 +591,29 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     int64_t mm = std::max<int64_t>(d.align_margin, bb);
     mm = std::min<int64_t>(mm, n - 2 - bb);
 
+    // resident workgroup if the system fits one, else the streaming path
+    // (FQSB_FORCE_STREAMING=1 selects the streaming path for every size)
+    if (g->minim != FQSB_MIN_OVERDAMPED) {
+        const char* force = std::getenv("FQSB_FORCE_STREAMING");
+        try {
+            if (force && force[0] == '1') {
+                throw Error("streaming forced");
+            }
+            g->choose_launch(0, 0);
+        }
+        catch (const Error&) {
+            g->streaming = true;
+            g->u2 = dalloc<double>(RN);
+            g->v2 = dalloc<double>(RN);
+            g->a2 = dalloc<double>(RN);
+            g->yl = dalloc<double>(RN);
+            g->yr = dalloc<double>(RN);
+            g->yl2 = dalloc<double>(RN);
+            g->yr2 = dalloc<double>(RN);
+            g->jl2 = dalloc<int32_t>(RN);
+        }
+    }
+
     g->hviews.resize((size_t)R);
     g->hrs.assign((size_t)R, fq::RunState{});
     for (int64_t r = 0; r < R; ++r) {
@@ -617,6 +658,16 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         S.stddev = d.stddev;
         S.seed = d.seed + (uint64_t)r * seed_stride;
         S.offset = d.offset;
+        if (g->streaming) {
+            S.u_b = g->u2 + o;
+            S.v_b = g->v2 + o;
+            S.a_b = g->a2 + o;
+            S.yl_a = g->yl + o;
+            S.yr_a = g->yr + o;
+            S.yl_b = g->yl2 + o;
+            S.yr_b = g->yr2 + o;
+            S.jl_b = g->jl2 + o;
+        }
         S.ls.y = g->y + o * (size_t)n;
         S.ls.start = g->start + o;
         S.ls.jloc = g->jloc + o;
@@ -645,9 +696,6 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     g->push();
     HIPC(fq::launch_landscape_init(g->views, (int)R, N, g->stream));
     g->refresh();
-    if (g->minim != FQSB_MIN_OVERDAMPED) {
-        g->choose_launch(0, 0);
-    }
     return g.release();
 }
 
@@ -1196,7 +1244,12 @@ uint64_t fqsb_batch_replicas(const fqsb_batch* b)
 
 int fqsb_batch_set_launch(fqsb_batch* b, int ppt, int threads)
 {
-    return guard([&] { b->g->choose_launch(ppt, threads); });
+    return guard([&] {
+        if (b->g->streaming) {
+            throw Error("the replicas are too large for a resident workgroup; they use the streaming path");
+        }
+        b->g->choose_launch(ppt, threads);
+    });
 }
 
 int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads)

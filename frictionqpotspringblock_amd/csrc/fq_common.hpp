@@ -125,6 +125,10 @@ struct SysView {
     int64_t lr_m;
     double* pref; // LongRange1d prefactor [N]
     double *u, *v, *a, *f, *f_pot, *f_int, *f_frame, *f_damp, *f_th;
+    // streaming path (systems larger than a resident workgroup): second state
+    // buffer and the yield positions of the current well of both buffers
+    double *u_b, *v_b, *a_b, *yl_a, *yr_a, *yl_b, *yr_b;
+    int32_t* jl_b;
     int64_t* i_ref; // reference well index for activity / truncate / events
     int64_t *next, *dinc;
     double mean, stddev; // thermal draw

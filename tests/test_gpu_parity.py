@@ -25,6 +25,17 @@ def fq():
     return fq
 
 
+@pytest.fixture(params=["resident", "streaming"])
+def path(request, monkeypatch):
+    """Run a test through the resident kernels and again through the streaming
+    kernel (selected for every size by FQSB_FORCE_STREAMING=1)."""
+    if request.param == "streaming":
+        monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
+    else:
+        monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
+    return request.param
+
+
 def assert_same_state(a, b, what=""):
     for name in FIELDS:
         x, y = getattr(a, name), getattr(b, name)
@@ -78,7 +89,7 @@ def _make(mod, sub, cls, n, extra, **kw):
 
 
 @pytest.mark.parametrize("sub,cls,n,extra", SYSTEMS, ids=[f"{s[1]}-{s[0]}-{s[2]}" for s in SYSTEMS])
-def test_dynamics_and_quasistatics(fq, oracle, sub, cls, n, extra):
+def test_dynamics_and_quasistatics(fq, oracle, path, sub, cls, n, extra):
     a = _make(fq, sub, cls, n, extra)
     b = _make(oracle, sub, cls, n, extra)
     assert_same_state(a, b, "construction")
@@ -117,7 +128,7 @@ def test_dynamics_and_quasistatics(fq, oracle, sub, cls, n, extra):
     assert np.array_equal(a.chunk.data, b.chunk.data)
 
 
-def test_until_event_and_truncate(fq, oracle):
+def test_until_event_and_truncate(fq, oracle, path):
     N = 500
     a = _make(fq, "Line1d", "System_Cuspy_Laplace", N, dict(k_interactions=1.0))
     b = _make(oracle, "Line1d", "System_Cuspy_Laplace", N, dict(k_interactions=1.0))
@@ -157,7 +168,7 @@ def test_nopassing(fq, oracle):
 
 
 @pytest.mark.parametrize("cls", ["System_Cuspy_Laplace_RandomForcing", "System_Cuspy_Quartic_RandomForcing"])
-def test_random_forcing(fq, oracle, cls):
+def test_random_forcing(fq, oracle, path, cls):
     N = 700
     extra = dict(k_interactions=1.0) if "Laplace" in cls else dict(a1=1.0, a2=5.0)
     rng = np.random.default_rng(0)
@@ -204,7 +215,7 @@ def test_reference_case(fq, case):
         ("Line1d_Cuspy_Laplace_LongRange", 6),
     ],
 )
-def test_golden_examples(fq, oracle, name, nstep):
+def test_golden_examples(fq, oracle, path, name, nstep):
     ref = P.golden(name)
     a = P.example_system(fq, name)
     ua, fa, Sa, ia = P.quasistatic(a, nstep)
@@ -338,3 +349,38 @@ def test_errors(fq):
         s.x
     with pytest.raises(RuntimeError, match="Deprecated"):
         s.f_neighbours
+
+
+# systems beyond the capacity of a resident workgroup (streaming path)
+LARGE = [
+    ("Line1d", "System_Cuspy_Laplace", 20000, dict(k_interactions=1.0), True),
+    ("Line1d", "System_SemiSmooth_Laplace", 12000, dict(k_interactions=1.0, kappa=1.0), True),
+    ("Line1d", "System_Cuspy_Quartic", 9000, dict(a1=1.0, a2=5.0), True),
+    ("Line2d", "System_Cuspy_Laplace", (128, 128), dict(k_interactions=1.0), True),
+    ("Line2d", "System_Cuspy_QuarticGradient", (100, 100), dict(k2=1.0, k4=1.0), False),
+    ("Line1d", "System_Cuspy_LongRange", 7000, dict(k_interactions=1.0, alpha=1.5), False),
+]
+
+
+@pytest.mark.parametrize("sub,cls,n,extra,minimise", LARGE, ids=[f"{s[1]}-{s[0]}-{s[2]}" for s in LARGE])
+def test_large_systems(fq, oracle, sub, cls, n, extra, minimise):
+    a = _make(fq, sub, cls, n, extra)
+    b = _make(oracle, sub, cls, n, extra)
+    assert_same_state(a, b, "construction")
+    scale = 1.0 if "Quartic" in cls else 10.0
+    v0 = np.random.default_rng(7).normal(0.0, scale, a.shape)
+    for s in (a, b):
+        s.v = v0
+    nsteps = 30 if "LongRange" in cls else 200
+    for s in (a, b):
+        s.flowSteps(nsteps, 0.2)
+    assert_same_state(a, b, "flowSteps")
+    assert np.array_equal(a.chunk.data, b.chunk.data)
+    if minimise:
+        assert a.minimise() == b.minimise() == 0
+        assert_same_state(a, b, "minimise")
+        for s in (a, b):
+            s.eventDrivenStep(1e-3, True)
+        assert a.minimise(time_activity=True) == b.minimise(time_activity=True) == 0
+        assert a.quasistaticActivityFirst == b.quasistaticActivityFirst
+        assert_same_state(a, b, "kick + minimise")
