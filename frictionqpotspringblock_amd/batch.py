@@ -40,6 +40,8 @@ DISTRIBUTION = {
     "weibull": 6,
     "normal": 7,
 }
+EXTERNAL = {None: 0, "RandomNormalForcing": 1}
+LONGRANGE_METHOD = {"exact": 0, "matrix": 1}
 FIELD = {
     "u": 0,
     "v": 1,
@@ -147,6 +149,8 @@ SIGNATURES = {
     "fqsb_batch_replicas": (C.c_uint64, [_P]),
     "fqsb_batch_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
     "fqsb_batch_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fqsb_batch_set_longrange_method": (C.c_int, [_P, C.c_int]),
+    "fqsb_batch_get_longrange_method": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "fqsb_batch_flow_steps": (C.c_int, [_P, C.c_uint64, C.c_double, C.c_uint64, _fp]),
     "fqsb_batch_time_steps": (C.c_int, [_P, C.c_uint64, C.c_uint64, _fp]),
     "fqsb_batch_minimise": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up]),
@@ -204,8 +208,20 @@ def make_desc(
     alpha=0.0,
     offset=-100.0,
     nchunk=5000,
+    external=None,
+    mean=0.0,
+    stddev=0.0,
+    seed_forcing=0,
+    dinc_init=None,
+    dinc=None,
 ):
-    """Build a ``fqsb_desc`` from the reference's constructor arguments."""
+    """Build a ``fqsb_desc`` from the reference's constructor arguments.
+
+    ``external="RandomNormalForcing"`` adds the random forcing of
+    Line1d.h:283-318 (``mean``, ``stddev``, ``seed_forcing``, ``dinc_init``,
+    ``dinc``; the two arrays have the system's size). The description keeps
+    references to the arrays it points to.
+    """
     if distribution not in DISTRIBUTION:
         raise RuntimeError("Unknown distribution: " + str(distribution))
     shape = [int(s) for s in shape]
@@ -230,6 +246,17 @@ def make_desc(
     d.nparameters = len(par)
     d.align_buffer = 2
     d.align_margin = 30
+    if external not in EXTERNAL:
+        raise RuntimeError("unknown external forcing: " + str(external))
+    d.external = EXTERNAL[external]
+    if d.external:
+        n = int(np.prod(shape))
+        di = np.ascontiguousarray(np.broadcast_to(np.asarray(dinc_init, dtype=np.int64), [n]))
+        dd = np.ascontiguousarray(np.broadcast_to(np.asarray(dinc, dtype=np.int64), [n]))
+        d._keep = (di, dd)
+        d.mean, d.stddev, d.seed_forcing = float(mean), float(stddev), int(seed_forcing)
+        d.dinc_init = di.ctypes.data_as(_ip)
+        d.dinc = dd.ctypes.data_as(_ip)
     return d
 
 
@@ -267,6 +294,21 @@ class Batch:
     def stream(self):
         """The hipStream_t (as an integer) the batch launches on."""
         return capi().fqsb_batch_stream(self._h)
+
+    @property
+    def longrange_method(self):
+        """How LongRange1d forces are evaluated: "exact" (the reference's
+        sequential sum) or "matrix" (circulant product on the matrix cores,
+        the default for Cuspy batches)."""
+        m = C.c_int()
+        _check(capi().fqsb_batch_get_longrange_method(self._h, C.byref(m)))
+        return {v: k for k, v in LONGRANGE_METHOD.items()}[m.value]
+
+    @longrange_method.setter
+    def longrange_method(self, name):
+        if name not in LONGRANGE_METHOD:
+            raise RuntimeError("unknown long-range method: " + str(name))
+        _check(capi().fqsb_batch_set_longrange_method(self._h, LONGRANGE_METHOD[name]))
 
     def set_launch(self, ppt=0, threads=0):
         _check(capi().fqsb_batch_set_launch(self._h, int(ppt), int(threads)))
@@ -357,6 +399,7 @@ class Batch:
     f_interactions = property(lambda self: self._get("f_interactions"))
     f_frame = property(lambda self: self._get("f_frame"))
     f_damping = property(lambda self: self._get("f_damping"))
+    f_thermal = property(lambda self: self._get("f_thermal"))
 
     @property
     def u_frame(self):

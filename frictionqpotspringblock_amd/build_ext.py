@@ -45,6 +45,7 @@ HIP_SOURCES = [
     "fq_capi.hip",
     "fq_service.hip",
     "fq_stream.hip",
+    "fq_longrange.hip",
     "fq_res_cuspy.hip",
     "fq_res_semismooth.hip",
     "fq_res_smooth.hip",

@@ -76,6 +76,8 @@ void dfree(T*& p)
 constexpr uint64_t MIN_BLOCK = 1u << 18;
 // steps per resident launch of timeSteps / flowSteps
 constexpr uint64_t STEP_BLOCK = 1u << 16;
+// steps between host checks of a batched long-range run (matrix form)
+constexpr uint64_t LR_BLOCK = 128;
 
 struct Group {
     fqsb_desc d{};
@@ -97,6 +99,11 @@ struct Group {
     double *u2 = nullptr, *v2 = nullptr, *a2 = nullptr, *yl = nullptr, *yr = nullptr, *yl2 = nullptr, *yr2 = nullptr;
     int32_t* jl2 = nullptr;
     size_t lds_limit = 65536;
+    // LongRange1d batches in matrix form (fq_longrange.hpp)
+    int lr_method = FQSB_LR_EXACT;
+    int64_t lr_m = 0;
+    double lr_c = 0.0;
+    double *lr_un = nullptr, *lr_fg = nullptr, *lr_yl = nullptr, *lr_yr = nullptr;
 
     Group() = default;
     Group(const Group&) = delete;
@@ -135,6 +142,10 @@ struct Group {
         dfree(yl2);
         dfree(yr2);
         dfree(jl2);
+        dfree(lr_un);
+        dfree(lr_fg);
+        dfree(lr_yl);
+        dfree(lr_yr);
         if (ev0) {
             (void)hipEventDestroy(ev0);
         }
@@ -358,6 +369,95 @@ struct Group {
         return ms;
     }
 
+    void set_lr_method(int method)
+    {
+        if (inter != FQSB_INT_LONGRANGE1D) {
+            throw Error("the systems have no long-range interactions");
+        }
+        if (method != FQSB_LR_EXACT && method != FQSB_LR_MATRIX) {
+            throw Error("unknown long-range method");
+        }
+        if (method == FQSB_LR_MATRIX && pot != FQSB_POT_CUSPY) {
+            throw Error("the matrix form of long-range interactions needs the Cuspy potential");
+        }
+        if (method == FQSB_LR_MATRIX && !lr_un) {
+            const size_t RN = (size_t)(R * N);
+            lr_un = dalloc<double>(RN);
+            lr_fg = dalloc<double>(RN);
+            lr_yl = dalloc<double>(RN);
+            lr_yr = dalloc<double>(RN);
+            // c = sum of one row of K: sum over d = 1 .. N-1 of pref[fold(d)]
+            std::vector<double> pr((size_t)N);
+            d2h(pr.data(), pref, (size_t)N);
+            double c = 0.0;
+            for (int64_t dd = 1; dd < N; ++dd) {
+                c += pr[(size_t)(dd > lr_m ? N - dd : dd)];
+            }
+            lr_c = c;
+        }
+        lr_method = method;
+    }
+
+    fq::LrBuffers lr_buffers() const
+    {
+        return fq::LrBuffers{lr_un, lr_fg, lr_yl, lr_yr, N, R, lr_c};
+    }
+
+    // timeSteps / flowSteps with LongRange1d in matrix form: per step one
+    // k_lr_gemm (interaction forces of all replicas) and one k_lr_elem (rest
+    // of timeStep). The host checks the run states every `block` steps; a
+    // replica whose next slips need a landscape refill stops (ST_REFILL)
+    // until k_refill and k_lr_prologue ran, then catches up (per-replica
+    // step target).
+    float lr_run(fq::RunArgs ra, uint64_t total, uint64_t block, bool timed)
+    {
+        reset_run(ra.mode);
+        for (auto& r : hrs) {
+            r.target = (int64_t)total;
+        }
+        push();
+        const fq::LrBuffers lb = lr_buffers();
+        if (timed) {
+            HIPC(hipEventRecord(ev0, stream));
+        }
+        HIPC(fq::launch_lr_prologue(views, lb, stream));
+        uint64_t todo = total;
+        while (todo > 0) {
+            const uint64_t nb = std::min<uint64_t>(block, todo);
+            for (uint64_t s = 0; s < nb; ++s) {
+                HIPC(fq::launch_lr_gemm(pref, lb, lr_m, stream));
+                HIPC(fq::launch_lr_elem(ext, views, lb, ra, stream));
+            }
+            pull();
+            bool refill = false;
+            int64_t lag = 0;
+            for (const auto& r : hrs) {
+                refill |= r.status == fq::ST_REFILL;
+                if (r.status == fq::ST_RUNNING || r.status == fq::ST_REFILL) {
+                    lag = std::max<int64_t>(lag, r.target - r.steps);
+                }
+            }
+            if (refill) {
+                HIPC(fq::launch_refill(views, (int)R, stream));
+                HIPC(fq::launch_lr_prologue(views, lb, stream));
+                for (auto& r : hrs) {
+                    if (r.status == fq::ST_REFILL) {
+                        r.status = fq::ST_RUNNING;
+                    }
+                }
+            }
+            todo = (uint64_t)lag;
+        }
+        float ms = 0.0f;
+        if (timed) {
+            HIPC(hipEventRecord(ev1, stream));
+            HIPC(hipEventSynchronize(ev1));
+            HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        }
+        pull();
+        return ms;
+    }
+
     float steps(uint64_t n, bool flow, double v_frame, uint64_t block, bool check_each, bool timed)
     {
         fq::RunArgs ra{};
@@ -365,7 +465,11 @@ struct Group {
         ra.flow = flow ? 1 : 0;
         ra.v_frame = v_frame;
         (void)check_each;
-        float ms = run(ra, n, block ? block : STEP_BLOCK, timed);
+        if (minim == FQSB_MIN_OVERDAMPED) {
+            throw Error("dynamics are not available for no-passing systems");
+        }
+        float ms = lr_method == FQSB_LR_MATRIX ? lr_run(ra, n, block ? block : LR_BLOCK, timed)
+                                               : run(ra, n, block ? block : STEP_BLOCK, timed);
         for (const auto& r : hrs) {
             if (r.status == fq::ST_NAN) {
                 throw Error("NaN entries found");
@@ -568,6 +672,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     int64_t lr_m = 0;
     if (inter == FQSB_INT_LONGRANGE1D) { // LongRange1d ctor detail.h:829-844
         lr_m = (N - N % 2) / 2;
+        g->lr_m = lr_m;
         std::vector<double> pr((size_t)N, 0.0);
         for (int64_t dd = 1; dd < N; ++dd) {
             pr[(size_t)dd] = d.k_interactions / std::pow((double)dd, d.alpha + 1.0);
@@ -1227,7 +1332,25 @@ int fqsb_batch_create(const fqsb_desc* desc, uint64_t replicas, uint64_t seed_st
         *out = nullptr;
         std::unique_ptr<fqsb_batch> b(new fqsb_batch());
         b->g.reset(build_group(desc, replicas, seed_stride));
+        if (b->g->inter == FQSB_INT_LONGRANGE1D && b->g->pot == FQSB_POT_CUSPY) {
+            b->g->set_lr_method(FQSB_LR_MATRIX);
+        }
         *out = b.release();
+    });
+}
+
+int fqsb_batch_set_longrange_method(fqsb_batch* b, int method)
+{
+    return guard([&] { b->g->set_lr_method(method); });
+}
+
+int fqsb_batch_get_longrange_method(const fqsb_batch* b, int* method)
+{
+    return guard([&] {
+        if (b->g->inter != FQSB_INT_LONGRANGE1D) {
+            throw Error("the systems have no long-range interactions");
+        }
+        *method = b->g->lr_method;
     });
 }
 

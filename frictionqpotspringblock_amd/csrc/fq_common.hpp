@@ -139,6 +139,18 @@ struct SysView {
 };
 
 // Uniform arguments of one resident launch.
+// Extra device pointers of a batched long-range group (all [R, N], row r =
+// replica r), used by the matrix-core form of LongRange1d.
+struct LrBuffers {
+    double* un; // slips of the step being computed
+    double* fg; // interaction force from the matrix product
+    double* yl; // current well, left
+    double* yr; // current well, right
+    int64_t N;
+    int64_t R;
+    double c; // sum of one row of the circulant matrix
+};
+
 struct RunArgs {
     int32_t mode;
     int32_t flow;          // flowSteps: u_frame += v_frame * dt before each step

@@ -204,9 +204,24 @@ uint64_t fqsb_batch_replicas(const fqsb_batch* b);
  * workgroup); 0, 0 restores the automatic choice. */
 int fqsb_batch_set_launch(fqsb_batch* b, int ppt, int threads);
 int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads);
+/* How a batch evaluates LongRange1d (detail.h:846-867):
+ * FQSB_LR_EXACT  -- the reference's sequential sum per particle, in the
+ *                   resident or streaming kernels; bit-identical to a single
+ *                   system of the same seed.
+ * FQSB_LR_MATRIX -- (default for Cuspy batches) the forces of all replicas as
+ *                   one product with the circulant matrix on the FP64 matrix
+ *                   cores; a different summation order, so each force differs
+ *                   from the sequential sum by a few units in the last place
+ *                   of the largest term. Used by timeSteps / flowSteps;
+ *                   minimise and the event-driven steps always use the exact
+ *                   sum. Fails on a batch without long-range interactions. */
+enum { FQSB_LR_EXACT = 0, FQSB_LR_MATRIX = 1 };
+int fqsb_batch_set_longrange_method(fqsb_batch* b, int method);
+int fqsb_batch_get_longrange_method(const fqsb_batch* b, int* method);
 /* flowSteps / timeSteps on all replicas, launched as resident blocks of
- * `block` steps (0: automatic). If ms is non-NULL it receives the time
- * between HIP events recorded on the launch stream around the launches. */
+ * `block` steps (0: automatic; with the long-range matrix form, the number of
+ * steps between checks of the run states). If ms is non-NULL it receives the
+ * time between HIP events recorded on the launch stream around the launches. */
 int fqsb_batch_flow_steps(fqsb_batch* b, uint64_t n, double v_frame, uint64_t block, float* ms);
 int fqsb_batch_time_steps(fqsb_batch* b, uint64_t n, uint64_t block, float* ms);
 /* minimise on all replicas; ret[r] as System::minimise. */

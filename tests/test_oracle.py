@@ -225,3 +225,46 @@ def test_errors(oracle):
         smooth.maxUniformDisplacement()
     with pytest.raises(RuntimeError, match="Unknown distribution: foo"):
         oracle.Line1d.System_Cuspy_Laplace(**P.delta_par(5, distribution="foo"))
+
+
+def circulant_longrange(u, k, alpha):
+    """LongRange1d forces (detail.h:829-867) through the eigen-form of the
+    circulant matrix: f = g (*) u - c u, with g[m] = k / fold(m)^(alpha+1)
+    (g[0] = 0), fold(m) = min(m, N - m), c = sum(g), (*) a circular
+    convolution evaluated by FFT. Rows of u are independent systems."""
+    u = np.atleast_2d(u)
+    N = u.shape[-1]
+    m = np.arange(N)
+    d = np.minimum(m, N - m).astype(float)
+    g = np.zeros(N)
+    g[1:] = k / d[1:] ** (alpha + 1.0)
+    conv = np.fft.irfft(np.fft.rfft(g) * np.fft.rfft(u, axis=-1), n=N, axis=-1)
+    return conv - g.sum() * u, g
+
+
+@pytest.mark.parametrize("N", [7, 10, 64, 301])
+def test_longrange_force_eigen_form(oracle, N):
+    """The oracle's sequential LongRange1d sum equals the FFT eigen-form and a
+    dense-matrix product, within the summation error bound gamma_N * sum|term|."""
+    rng = np.random.default_rng(N)
+    k, alpha = 0.7, 1.5
+    s = oracle.Line1d.System_Cuspy_LongRange(
+        m=1.0, eta=0.1, mu=1.0, k_interactions=k, alpha=alpha, k_frame=0.1, dt=0.1, shape=[N], seed=0,
+        distribution="random", parameters=[2.0], offset=-5.0, nchunk=100,
+    )
+    u = rng.normal(size=N) * 3.0
+    s.u = u
+    f_oracle = s.f_interactions
+    f_fft, g = circulant_longrange(u, k, alpha)
+    idx = np.arange(N)
+    K = g[(idx[:, None] - idx[None, :]) % N]
+    f_dense = K @ u - g.sum() * u
+    scale = np.abs(K) @ np.abs(u) + g.sum() * np.abs(u)
+    gamma = 4 * N * np.finfo(float).eps
+    assert np.all(np.abs(f_oracle - f_dense) <= gamma * scale)
+    assert np.all(np.abs(f_oracle - f_fft[0]) <= gamma * scale * np.log2(N) + 1e-15 * np.max(scale))
+    # a dropped self term or a wrong fold would fail: perturb and recheck
+    g_bad = g.copy()
+    g_bad[1] *= 1.5
+    K_bad = g_bad[(idx[:, None] - idx[None, :]) % N]
+    assert not np.allclose(K_bad @ u - g_bad.sum() * u, f_oracle)
