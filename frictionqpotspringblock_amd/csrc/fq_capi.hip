@@ -103,7 +103,8 @@ struct Group {
     int lr_method = FQSB_LR_EXACT;
     int64_t lr_m = 0;
     double lr_c = 0.0;
-    double *lr_un = nullptr, *lr_fg = nullptr, *lr_yl = nullptr, *lr_yr = nullptr;
+    int lr_nsplit = 1;
+    double *lr_un = nullptr, *lr_part = nullptr, *lr_yl = nullptr, *lr_yr = nullptr;
 
     Group() = default;
     Group(const Group&) = delete;
@@ -143,7 +144,7 @@ struct Group {
         dfree(yr2);
         dfree(jl2);
         dfree(lr_un);
-        dfree(lr_fg);
+        dfree(lr_part);
         dfree(lr_yl);
         dfree(lr_yr);
         if (ev0) {
@@ -383,7 +384,20 @@ struct Group {
         if (method == FQSB_LR_MATRIX && !lr_un) {
             const size_t RN = (size_t)(R * N);
             lr_un = dalloc<double>(RN);
-            lr_fg = dalloc<double>(RN);
+            // split the summation index so that one product launch has about
+            // 1024 workgroups (4 per CU, the occupancy that registers and LDS
+            // allow), each range at least 8 K-blocks long; FQSB_LR_SPLIT
+            // overrides the count (tuning)
+            const int64_t kb = fq::lr_split_blocks();
+            const int64_t nkb = (N + kb - 1) / kb;
+            const int64_t tiles = ((N + 127) / 128) * ((R + 63) / 64);
+            const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(8, nkb / 8));
+            int64_t ns = std::max<int64_t>(1, std::min<int64_t>(cap, (1024 + tiles - 1) / tiles));
+            if (const char* e = std::getenv("FQSB_LR_SPLIT")) {
+                ns = std::max<int64_t>(1, std::min<int64_t>(cap, std::atoll(e)));
+            }
+            lr_nsplit = (int)ns;
+            lr_part = dalloc<double>(RN * (size_t)ns);
             lr_yl = dalloc<double>(RN);
             lr_yr = dalloc<double>(RN);
             // c = sum of one row of K: sum over d = 1 .. N-1 of pref[fold(d)]
@@ -400,7 +414,7 @@ struct Group {
 
     fq::LrBuffers lr_buffers() const
     {
-        return fq::LrBuffers{lr_un, lr_fg, lr_yl, lr_yr, N, R, lr_c};
+        return fq::LrBuffers{lr_un, lr_part, lr_yl, lr_yr, N, R, lr_c, lr_nsplit};
     }
 
     // timeSteps / flowSteps with LongRange1d in matrix form: per step one

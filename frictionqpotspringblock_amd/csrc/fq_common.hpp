@@ -142,13 +142,14 @@ struct SysView {
 // Extra device pointers of a batched long-range group (all [R, N], row r =
 // replica r), used by the matrix-core form of LongRange1d.
 struct LrBuffers {
-    double* un; // slips of the step being computed
-    double* fg; // interaction force from the matrix product
-    double* yl; // current well, left
-    double* yr; // current well, right
+    double* un;   // slips of the step being computed
+    double* part; // [nsplit][R][N] partial products K[:, I_s] (un - ub)[I_s]
+    double* yl;   // current well, left
+    double* yr;   // current well, right
     int64_t N;
     int64_t R;
-    double c; // sum of one row of the circulant matrix
+    double c;   // sum of one row of the circulant matrix
+    int nsplit; // number of partial products
 };
 
 struct RunArgs {

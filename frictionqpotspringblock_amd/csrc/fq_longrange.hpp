@@ -12,10 +12,13 @@
 // within a relative tolerance, not bit for bit.
 //
 // One step of the batched run is two kernels:
-//   k_lr_gemm : fg = K (un - ub) - c (un - ub)  for the slips un of the step
-//   k_lr_elem : well walk for un (or a refill request), potential, frame,
-//               thermal draws, the three velocity passes of timeStep, and the
-//               slips of the next step un = u + dt v + dt^2/2 a.
+//   k_lr_gemm : part[s] = K[:, I_s] (un - ub)[I_s] for the slips un of the
+//               step, one partial product per range I_s of the summation
+//               index (split-K, so that a launch fills the chip)
+//   k_lr_elem : fg = sum_s part[s] (fixed order) - c (un - ub); well walk for
+//               un (or a refill request), potential, frame, thermal draws,
+//               the three velocity passes of timeStep, and the slips of the
+//               next step un = u + dt v + dt^2/2 a.
 #pragma once
 
 #include "fq_resident.hpp"
@@ -28,6 +31,10 @@ constexpr int LR_BM = 128; // rows (particles) per workgroup
 constexpr int LR_BN = 64;  // columns (replicas) per workgroup
 constexpr int LR_BK = 32;  // K-block
 constexpr int LR_THREADS = 256;
+constexpr int LR_GW = LR_BM + LR_BK;            // window of g per K-block
+constexpr int LR_BPT = LR_BN * LR_BK / LR_THREADS; // B values per thread per K-block (8)
+static_assert(LR_BPT * 4 == LR_BK && LR_THREADS / 4 == LR_BN, "B staging: 4 threads per replica column");
+constexpr int LR_XCDS = 8;                      // XCDs of an MI355X
 
 FQ_HD int64_t lr_fold(int64_t m, int64_t N, int64_t lr_m)
 {
@@ -35,27 +42,85 @@ FQ_HD int64_t lr_fold(int64_t m, int64_t N, int64_t lr_m)
     return a > lr_m ? N - a : a;
 }
 
-// fg[r][p] for a BM x BN tile. 4 wavefronts in 2 x 2; each computes a 64 x 32
-// sub-tile as 4 x 2 MFMA 16x16 tiles. Lane layout of v_mfma_f64_16x16x4f64:
-// A[i = l%16][k = l/16], B[k = l/16][j = l%16], D[i = 4v + l/16][j = l%16].
+// Launch geometry of k_lr_gemm: tm x tn output tiles, nsplit ranges of the
+// summation index of klen (a multiple of LR_BK) each.
+struct LrGeom {
+    int tm, tn, nsplit;
+    int64_t klen;
+};
+
+// part[s][r][p] for one BM x BN tile and one summation range. 4 wavefronts in
+// 2 x 2; each computes a 64 x 32 sub-tile as 4 x 2 MFMA 16x16 tiles. Lane
+// layout of v_mfma_f64_16x16x4f64: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+// D[i = 4v + l/16][j = l%16]. The next K-block is loaded into registers while
+// the current one is multiplied (two LDS buffers, one barrier per K-block).
+// Workgroups are dispatched round-robin over the XCDs; the linear id is
+// remapped so that each XCD works on contiguous row tiles of the same replica
+// columns and summation range, whose B operand then stays in that XCD's L2.
 __global__ void __launch_bounds__(LR_THREADS) k_lr_gemm(const double* __restrict__ pref, LrBuffers lb,
-                                                         int64_t lr_m)
+                                                         int64_t lr_m, LrGeom geo)
 {
-    __shared__ double s_g[LR_BM + LR_BK];
-    __shared__ double s_b[LR_BN][LR_BK + 1];
-    __shared__ double s_ub[LR_BN];
+    __shared__ double s_g[2][LR_GW];
+    __shared__ double s_b[2][LR_BN][LR_BK + 1];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t N = lb.N, R = lb.R;
-    const int64_t p0 = (int64_t)blockIdx.x * LR_BM;
-    const int64_t r0 = (int64_t)blockIdx.y * LR_BN;
+
+    const int total = geo.tm * geo.tn * geo.nsplit;
+    int id = (int)blockIdx.x;
+    if (total % LR_XCDS == 0) {
+        id = (id % LR_XCDS) * (total / LR_XCDS) + id / LR_XCDS;
+    }
+    const int rb = id % geo.tm;
+    const int cb = (id / geo.tm) % geo.tn;
+    const int sp = id / (geo.tm * geo.tn);
+
+    const int64_t p0 = (int64_t)rb * LR_BM;
+    const int64_t r0 = (int64_t)cb * LR_BN;
+    const int64_t kbeg = (int64_t)sp * geo.klen;
+    const int64_t kend = kbeg + geo.klen < N ? kbeg + geo.klen : N;
+    const int nkb = (int)((kend - kbeg + LR_BK - 1) / LR_BK);
     const int wr = wave >> 1, wc = wave & 1; // wave position in the 2 x 2 grid
     const int li = lane & 15, lk = lane >> 4;
 
-    if (t < LR_BN) {
-        const int64_t r = r0 + t;
-        s_ub[t] = r < R ? lb.un[r * N] : 0.0;
-    }
+    // register staging of one K-block: the g window value (threads < LR_GW)
+    // and LR_BPT values of B - ub: replica column bj = t / 4, summation
+    // indices bk0 .. bk0 + 7 (bk0 = (t % 4) * 8); zero outside the matrix and
+    // outside the summation range
+    const int bj = t >> 2, bk0 = (t & 3) * LR_BPT;
+    const int64_t br = r0 + bj;
+    const bool bvalid = br < R;
+    const double* const brow = lb.un + (bvalid ? br : 0) * N;
+    const double ubt = bvalid ? brow[0] : 0.0;
+    double rg = 0.0, rbv[LR_BPT];
+    auto load = [&](int64_t i0) {
+        if (t < LR_GW) {
+            const int64_t m = p0 - i0 - (LR_BK - 1) + t;
+            rg = (m > -N && m < N) ? pref[lr_fold(m, N, lr_m)] : 0.0;
+        }
+        const int64_t ib = i0 + bk0;
+        if (bvalid && ib + LR_BPT <= kend) {
+#pragma unroll
+            for (int q = 0; q < LR_BPT; ++q) {
+                rbv[q] = brow[ib + q] - ubt;
+            }
+        }
+        else {
+#pragma unroll
+            for (int q = 0; q < LR_BPT; ++q) {
+                rbv[q] = (bvalid && ib + q < kend) ? brow[ib + q] - ubt : 0.0;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        if (t < LR_GW) {
+            s_g[buf][t] = rg;
+        }
+#pragma unroll
+        for (int q = 0; q < LR_BPT; ++q) {
+            s_b[buf][bj][bk0 + q] = rbv[q];
+        }
+    };
 
     lr_double4 acc[4][2];
 #pragma unroll
@@ -65,26 +130,17 @@ __global__ void __launch_bounds__(LR_THREADS) k_lr_gemm(const double* __restrict
             acc[a][b] = lr_double4{0.0, 0.0, 0.0, 0.0};
         }
     }
+
+    load(kbeg);
+    store(0);
     __syncthreads();
 
-    for (int64_t i0 = 0; i0 < N; i0 += LR_BK) {
-        // window g[m] for m = p - i in (p0 - i0 - BK, p0 - i0 + BM):
-        // s_g[x] = g(p0 - i0 - (BK - 1) + x)
-        for (int x = t; x < LR_BM + LR_BK; x += LR_THREADS) {
-            const int64_t m = p0 - i0 - (LR_BK - 1) + x;
-            double g = 0.0;
-            if (m > -N && m < N) {
-                g = pref[lr_fold(m, N, lr_m)];
-            }
-            s_g[x] = g;
+    for (int kb = 0; kb < nkb; ++kb) {
+        const int buf = kb & 1;
+        const bool more = kb + 1 < nkb;
+        if (more) {
+            load(kbeg + (int64_t)(kb + 1) * LR_BK);
         }
-        // B tile: s_b[j][k] = un[r0 + j][i0 + k] - ub[j]
-        for (int x = t; x < LR_BN * LR_BK; x += LR_THREADS) {
-            const int j = x / LR_BK, k = x % LR_BK;
-            const int64_t r = r0 + j, i = i0 + k;
-            s_b[j][k] = (r < R && i < N) ? lb.un[r * N + i] - s_ub[j] : 0.0;
-        }
-        __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < LR_BK; ks += 4) {
             double af[4], bf[2];
@@ -92,11 +148,11 @@ __global__ void __launch_bounds__(LR_THREADS) k_lr_gemm(const double* __restrict
             for (int a = 0; a < 4; ++a) {
                 // row p = p0 + wr*64 + a*16 + li, column i = i0 + ks + lk
                 const int row = wr * 64 + a * 16 + li;
-                af[a] = s_g[row - (ks + lk) + (LR_BK - 1)];
+                af[a] = s_g[buf][row - (ks + lk) + (LR_BK - 1)];
             }
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                bf[b] = s_b[wc * 32 + b * 16 + li][ks + lk];
+                bf[b] = s_b[buf][wc * 32 + b * 16 + li][ks + lk];
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
@@ -106,22 +162,23 @@ __global__ void __launch_bounds__(LR_THREADS) k_lr_gemm(const double* __restrict
                 }
             }
         }
+        if (more) {
+            store(buf ^ 1);
+        }
         __syncthreads();
     }
 
-    // epilogue: fg[r][p] = acc - c (un[r][p] - ub[r])
+    double* const part = lb.part + (size_t)sp * (size_t)(R * N);
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int jl = wc * 32 + b * 16 + li;
-            const int64_t r = r0 + jl;
+            const int64_t r = r0 + wc * 32 + b * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int64_t p = p0 + wr * 64 + a * 16 + 4 * v + lk;
                 if (r < R && p < N) {
-                    const double up = lb.un[r * N + p] - s_ub[jl];
-                    lb.fg[r * N + p] = acc[a][b][v] - lb.c * up;
+                    part[r * N + p] = acc[a][b][v];
                 }
             }
         }
@@ -129,7 +186,7 @@ __global__ void __launch_bounds__(LR_THREADS) k_lr_gemm(const double* __restrict
 }
 
 // The rest of one step for every replica (one workgroup per replica), after
-// k_lr_gemm produced fg for the slips un of this step.
+// k_lr_gemm produced the partial products for the slips un of this step.
 template <int EXT>
 __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ views, LrBuffers lb, RunArgs ra)
 {
@@ -148,6 +205,7 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
     const int64_t min_next = EXT ? uniform_ll(rs->min_next) : 0;
     const uint64_t rng0 = EXT ? (uint64_t)uniform_ll((long long)rs->rng_state) : 0;
     const uint64_t rng_inc = EXT ? (uint64_t)uniform_ll((long long)rs->rng_inc) : 0;
+    const double ub = lb.un[(int64_t)blockIdx.x * S.N]; // the shift of the product (un[0] is rewritten below)
     __shared__ int s_flag[2];
     __shared__ int s_scan_w[RES_MAX_WAVES];
     __shared__ long long s_min_w[RES_MAX_WAVES];
@@ -160,7 +218,10 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
     const int32_t jmin = (int32_t)L.buffer;
     const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
     double* const un = lb.un + off;
-    const double* const fg = lb.fg + off;
+    const double* const part = lb.part + off;
+    const size_t pstride = (size_t)(lb.R * N);
+    const int nsplit = lb.nsplit;
+    const double c = lb.c;
     double* const yl = lb.yl + off;
     double* const yr = lb.yr + off;
 
@@ -248,7 +309,11 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
             yl[p] = lo;
             yr[p] = hi;
         }
-        const double fi = fg[p];
+        double acc = part[p];
+        for (int sp = 1; sp < nsplit; ++sp) {
+            acc += part[(size_t)sp * pstride + p];
+        }
+        const double fi = acc - c * (x - ub);
         const double fp = potential_force<POT_CUSPY>(S, x, lo, hi);
         const double ff = kf * (uf - x);
         const double sum3 = ff + fp + fi;
