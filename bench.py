@@ -1,17 +1,24 @@
 """Benchmark of the dynamics hot path on one MI355X.
 
-Workload (the batched-flow configuration): R = 1024 independent Line1d
-Cuspy-Laplace replicas of N = 4096 particles (replica r built with seed r),
-FP64, m = 1, eta = 2 sqrt(3) / 10, mu = 1, k_interactions = 1, k_frame = 1/N,
-dt = 0.1, Weibull(k=2, scale=1) yield increments in chunks of 200 wells,
-driven by flowSteps at v_frame = 1e-3. One bench step = one resident launch of
-1000 flowSteps over all replicas (4.19e9 particle-steps); the default
---steps 100 is the full 1e5-step run.
+Default workload (--workload flow, the batched-flow configuration): R = 1024
+independent Line1d Cuspy-Laplace replicas of N = 4096 particles (replica r
+built with seed r), FP64, m = 1, eta = 2 sqrt(3) / 10, mu = 1,
+k_interactions = 1, k_frame = 1/N, dt = 0.1, Weibull(k=2, scale=1) yield
+increments in chunks of 200 wells, driven by flowSteps at v_frame = 1e-3. One
+bench step = one resident launch of 1000 flowSteps over all replicas (4.19e9
+particle-steps); the default --steps 100 is the full 1e5-step run.
+
+--workload longrange: R = 256 replicas of N = 8192 particles with LongRange1d
+interactions (alpha = 1.5, k = 1) and random normal forcing (mean 0, stddev
+0.1, dinc = dinc_init = 10), otherwise as above; timeSteps with the
+interaction forces of all replicas as one FP64 matrix-core product per step.
+One bench step = 100 timeSteps (5.4e7 particle-steps); the default --steps
+200 is the full 2e4-step run.
 
 Prints one JSON line: whole-job particle-steps/s of the batched path, the
-plain one-system path beside it, the roofline of the resident kernel (FP64
-vector ALU, 40 algorithmic FLOP per particle-step, timed with HIP events on
-the launch stream) and the CPU oracle timed on a bounded sample.
+plain one-system path beside it, the roofline of the dominant kernel (timed
+with HIP events on the launch stream) and the CPU oracle timed on a bounded
+sample.
 """
 
 import argparse
@@ -30,8 +37,12 @@ R = 1024
 N = 4096
 BLOCK = 1000
 V_FRAME = 1e-3
-FLOP_PER_PARTICLE_STEP = 40.0  # Cuspy + Laplace1d Velocity-Verlet step, see DESIGN.md
+FLOP_PER_PARTICLE_STEP = 40.0  # Cuspy + Laplace1d Velocity-Verlet step
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+
+LR_R = 256
+LR_N = 8192
+LR_BLOCK = 100
 METRIC = "particle-steps/sec batched vs plain on 1 GPU (FP64) + % HBM/MFMA roofline"
 
 PARAMS = dict(
@@ -47,6 +58,19 @@ PARAMS = dict(
     parameters=[2.0, 1.0, 0.0],
     offset=-100.0,
     nchunk=200,
+)
+
+
+LR_PARAMS = dict(
+    PARAMS,
+    k_frame=1.0 / LR_N,
+    shape=[LR_N],
+    alpha=1.5,
+    mean=0.0,
+    stddev=0.1,
+    seed_forcing=0,
+    dinc_init=10,
+    dinc=10,
 )
 
 
@@ -94,23 +118,121 @@ def plain_path(steps=20000):
     return N * steps / (time.perf_counter() - t0)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100, help="timed resident blocks of 1000 flowSteps")
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")
-    ap.add_argument("--no-autotune", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(
-            "bench.py measures one GPU: replicas are independent and would shard across GPUs "
-            "without communication; run with --gpus 1",
-            file=sys.stderr,
-        )
-        sys.exit(2)
+def lr_cpu_baseline(replicas=16, target_s=10.0):
+    """The oracle on the host for the long-range workload: `replicas` systems
+    (one per thread) of the same physics, timeSteps sized from a pilot to take
+    about `target_s`. Each step is the reference's O(N^2) sequential sum."""
+    from oracle import pyoracle
 
+    pyoracle.lib()
+    cores = min(16, os.cpu_count() or 1)
+    replicas = max(replicas, cores)
+    kw = dict(LR_PARAMS)
+    systems = [
+        pyoracle._System(
+            rank=1, potential="Cuspy", interactions="LongRange1d", external=1, minimisation=2,
+            **dict(kw, seed=r * LR_N, seed_forcing=r * LR_N),
+        )
+        for r in range(replicas)
+    ]
+
+    def sweep(n):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+            list(ex.map(lambda s: s.timeSteps(n), systems))
+        return time.perf_counter() - t0
+
+    pilot = 2
+    dt_pilot = sweep(pilot)
+    steps = max(pilot, int(pilot * target_s / max(dt_pilot, 1e-3)))
+    dt = sweep(steps)
+    return {
+        "value": replicas * LR_N * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{replicas} replicas x N={LR_N} x {steps} timeSteps of the same workload, "
+        f"one replica per thread ({dt:.1f} s)",
+    }
+
+
+def lr_batch(replicas, method):
+    from frictionqpotspringblock_amd.batch import Batch
+
+    b = Batch(
+        replicas, seed_stride=LR_N, potential="Cuspy", interactions="LongRange1d",
+        external="RandomNormalForcing", **LR_PARAMS,
+    )
+    b.longrange_method = method
+    return b
+
+
+def lr_plain_path(steps=200):
+    """One system per launch: the reference's sequential long-range sum."""
+    b = lr_batch(1, "exact")
+    b.timeSteps(10)
+    ms = b.timeSteps(steps, timed=True)
+    return LR_N * steps / (ms * 1e-3)
+
+
+def run_longrange(args):
+    import torch
+
+    torch.cuda.init()
+    batch = lr_batch(LR_R, "matrix")
+    for _ in range(args.warmup):
+        batch.timeSteps(LR_BLOCK, block=LR_BLOCK)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = batch.timeSteps(args.steps * LR_BLOCK, block=LR_BLOCK, timed=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert np.all(np.isfinite(batch.u[:, :8]))
+    product_ms = batch.longrange_product_ms
+    steps_total = args.steps * LR_BLOCK
+    value = LR_R * LR_N * steps_total / wall
+    flop = 2.0 * LR_N * LR_N * LR_R  # per product launch
+    achieved = flop / (product_ms * 1e-3) / 1e12
+    plain = None if args.no_plain else lr_plain_path()
+    cpu = None if args.no_cpu else lr_cpu_baseline()
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Weibull yield landscape and normal forcing from pcg32 seeds r*{LR_N})",
+        "config": {
+            "workload": f"batched long-range: {LR_R} Line1d Cuspy-LongRange(alpha=1.5) + random-forcing "
+            f"replicas x N={LR_N}, timeSteps in blocks of {LR_BLOCK}",
+            "replicas": LR_R,
+            "N": LR_N,
+            "time_steps": steps_total,
+            "events_ms": ev_ms,
+        },
+        "plain_value": plain,
+        "batched_over_plain": None if plain is None else value / plain,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "k_lr_gemm",
+            "kernel_ms": product_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
+def run_flow(args):
     import torch
 
     from frictionqpotspringblock_amd.batch import Batch
@@ -139,7 +261,7 @@ def main():
     plain = None if args.no_plain else plain_path()
     cpu = None if args.no_cpu else cpu_baseline()
     ppt, threads = batch.launch
-    line = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
@@ -176,6 +298,32 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["flow", "longrange"], default="flow")
+    ap.add_argument(
+        "--steps", type=int, default=None,
+        help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
+        "longrange: blocks of 100 timeSteps, default 200)",
+    )
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: replicas are independent and would shard across GPUs "
+            "without communication; run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = 100 if args.workload == "flow" else 200
+    line = run_flow(args) if args.workload == "flow" else run_longrange(args)
     print(json.dumps(line), flush=True)
 
 

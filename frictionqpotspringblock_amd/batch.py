@@ -151,6 +151,7 @@ SIGNATURES = {
     "fqsb_batch_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fqsb_batch_set_longrange_method": (C.c_int, [_P, C.c_int]),
     "fqsb_batch_get_longrange_method": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "fqsb_batch_longrange_product_ms": (C.c_int, [_P, _fp]),
     "fqsb_batch_flow_steps": (C.c_int, [_P, C.c_uint64, C.c_double, C.c_uint64, _fp]),
     "fqsb_batch_time_steps": (C.c_int, [_P, C.c_uint64, C.c_uint64, _fp]),
     "fqsb_batch_minimise": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up]),
@@ -309,6 +310,14 @@ class Batch:
         if name not in LONGRANGE_METHOD:
             raise RuntimeError("unknown long-range method: " + str(name))
         _check(capi().fqsb_batch_set_longrange_method(self._h, LONGRANGE_METHOD[name]))
+
+    @property
+    def longrange_product_ms(self):
+        """Average duration (ms) of one long-range product launch in the last
+        timed timeSteps / flowSteps call of the matrix form."""
+        ms = C.c_float()
+        _check(capi().fqsb_batch_longrange_product_ms(self._h, C.byref(ms)))
+        return ms.value
 
     def set_launch(self, ppt=0, threads=0):
         _check(capi().fqsb_batch_set_launch(self._h, int(ppt), int(threads)))

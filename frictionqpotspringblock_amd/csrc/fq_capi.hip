@@ -104,6 +104,8 @@ struct Group {
     int64_t lr_m = 0;
     double lr_c = 0.0;
     int lr_nsplit = 1;
+    hipEvent_t evg0 = nullptr, evg1 = nullptr; // around one product launch per block (timed runs)
+    float lr_product_ms = 0.0f;                // average product duration of the last timed run
     double *lr_un = nullptr, *lr_part = nullptr, *lr_yl = nullptr, *lr_yr = nullptr;
 
     Group() = default;
@@ -152,6 +154,12 @@ struct Group {
         }
         if (ev1) {
             (void)hipEventDestroy(ev1);
+        }
+        if (evg0) {
+            (void)hipEventDestroy(evg0);
+        }
+        if (evg1) {
+            (void)hipEventDestroy(evg1);
         }
         if (stream) {
             (void)hipStreamDestroy(stream);
@@ -431,6 +439,12 @@ struct Group {
         }
         push();
         const fq::LrBuffers lb = lr_buffers();
+        if (timed && !evg0) {
+            HIPC(hipEventCreate(&evg0));
+            HIPC(hipEventCreate(&evg1));
+        }
+        double product_ms = 0.0;
+        int64_t product_n = 0;
         if (timed) {
             HIPC(hipEventRecord(ev0, stream));
         }
@@ -439,10 +453,24 @@ struct Group {
         while (todo > 0) {
             const uint64_t nb = std::min<uint64_t>(block, todo);
             for (uint64_t s = 0; s < nb; ++s) {
+                // timed runs: the first product of each block between two events
+                const bool probe = timed && s == 0;
+                if (probe) {
+                    HIPC(hipEventRecord(evg0, stream));
+                }
                 HIPC(fq::launch_lr_gemm(pref, lb, lr_m, stream));
+                if (probe) {
+                    HIPC(hipEventRecord(evg1, stream));
+                }
                 HIPC(fq::launch_lr_elem(ext, views, lb, ra, stream));
             }
             pull();
+            if (timed) {
+                float e = 0.0f;
+                HIPC(hipEventElapsedTime(&e, evg0, evg1));
+                product_ms += e;
+                ++product_n;
+            }
             bool refill = false;
             int64_t lag = 0;
             for (const auto& r : hrs) {
@@ -467,6 +495,7 @@ struct Group {
             HIPC(hipEventRecord(ev1, stream));
             HIPC(hipEventSynchronize(ev1));
             HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+            lr_product_ms = product_n ? (float)(product_ms / (double)product_n) : 0.0f;
         }
         pull();
         return ms;
@@ -807,7 +836,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         rs.prev = std::numeric_limits<double>::infinity();
         rs.status = fq::ST_RUNNING;
         uint64_t rinc = 0;
-        rs.rng_state = default_stream_state(d.seed_forcing, rinc);
+        rs.rng_state = default_stream_state(d.seed_forcing + (uint64_t)r * seed_stride, rinc);
         rs.rng_inc = rinc;
         rs.min_next = std::numeric_limits<int64_t>::max();
     }
@@ -1356,6 +1385,16 @@ int fqsb_batch_create(const fqsb_desc* desc, uint64_t replicas, uint64_t seed_st
 int fqsb_batch_set_longrange_method(fqsb_batch* b, int method)
 {
     return guard([&] { b->g->set_lr_method(method); });
+}
+
+int fqsb_batch_longrange_product_ms(const fqsb_batch* b, float* ms)
+{
+    return guard([&] {
+        if (b->g->inter != FQSB_INT_LONGRANGE1D) {
+            throw Error("the systems have no long-range interactions");
+        }
+        *ms = b->g->lr_product_ms;
+    });
 }
 
 int fqsb_batch_get_longrange_method(const fqsb_batch* b, int* method)

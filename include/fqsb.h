@@ -191,9 +191,11 @@ int fqsb_thermal_set_next(fqsb_system* s, const int64_t* in);
 /* ------------------------------------------------------------------------
  * Batched form (no reference counterpart: the reference runs one system per
  * object). R replicas of one description; replica r uses landscape seed
- * desc->seed + r * seed_stride, so with seed_stride = 1 replica r equals the
- * single system constructed with seed = desc->seed + r. All replicas advance
- * together in one resident launch (one workgroup per replica).
+ * desc->seed + r * seed_stride (and, with random forcing, forcing seed
+ * desc->seed_forcing + r * seed_stride), so with seed_stride = 1 replica r
+ * equals the single system constructed with seed = desc->seed + r (and
+ * seed_forcing = desc->seed_forcing + r). All replicas advance together in
+ * one resident launch (one workgroup per replica).
  * ---------------------------------------------------------------------- */
 typedef struct fqsb_batch fqsb_batch;
 
@@ -218,6 +220,10 @@ int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads);
 enum { FQSB_LR_EXACT = 0, FQSB_LR_MATRIX = 1 };
 int fqsb_batch_set_longrange_method(fqsb_batch* b, int method);
 int fqsb_batch_get_longrange_method(const fqsb_batch* b, int* method);
+/* Average duration (ms) of one matrix-product launch in the last timed
+ * timeSteps / flowSteps call of the matrix form (HIP events around the first
+ * product of every block, on the launch stream); 0 if none was timed. */
+int fqsb_batch_longrange_product_ms(const fqsb_batch* b, float* ms);
 /* flowSteps / timeSteps on all replicas, launched as resident blocks of
  * `block` steps (0: automatic; with the long-range matrix form, the number of
  * steps between checks of the run states). If ms is non-NULL it receives the
