@@ -41,6 +41,7 @@ DISTRIBUTION = {
     "normal": 7,
 }
 EXTERNAL = {None: 0, "RandomNormalForcing": 1}
+MINIMISATION = {"Inertial": 0, "Overdamped": 1, "None": 2}
 LONGRANGE_METHOD = {"exact": 0, "matrix": 1}
 FIELD = {
     "u": 0,
@@ -162,6 +163,26 @@ SIGNATURES = {
     "fqsb_batch_get_inc": (C.c_int, [_P, _ip]),
     "fqsb_batch_chunk_index_at_align": (C.c_int, [_P, _ip]),
     "fqsb_batch_stream": (C.c_void_p, [_P]),
+    "fqsb_ragged_create": (C.c_int, [C.POINTER(Desc), C.c_uint64, C.POINTER(_P)]),
+    "fqsb_ragged_destroy": (C.c_int, [_P]),
+    "fqsb_ragged_systems": (C.c_uint64, [_P]),
+    "fqsb_ragged_groups": (C.c_uint64, [_P]),
+    "fqsb_ragged_sizes": (C.c_int, [_P, _up]),
+    "fqsb_ragged_time_steps": (C.c_int, [_P, C.c_uint64]),
+    "fqsb_ragged_flow_steps": (C.c_int, [_P, C.c_uint64, C.c_double]),
+    "fqsb_ragged_minimise": (
+        C.c_int,
+        [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint8), _up],
+    ),
+    "fqsb_ragged_event_driven_step": (
+        C.c_int,
+        [_P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_uint8), _dp],
+    ),
+    "fqsb_ragged_get": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_ragged_set": (C.c_int, [_P, C.c_int, _dp]),
+    "fqsb_ragged_get_u_frame": (C.c_int, [_P, _dp]),
+    "fqsb_ragged_get_inc": (C.c_int, [_P, _ip]),
+    "fqsb_ragged_chunk_index_at_align": (C.c_int, [_P, _ip]),
 }
 
 _lib = None
@@ -210,6 +231,7 @@ def make_desc(
     offset=-100.0,
     nchunk=5000,
     external=None,
+    minimisation="Inertial",
     mean=0.0,
     stddev=0.0,
     seed_forcing=0,
@@ -247,6 +269,9 @@ def make_desc(
     d.nparameters = len(par)
     d.align_buffer = 2
     d.align_margin = 30
+    if minimisation not in MINIMISATION:
+        raise RuntimeError("unknown minimisation: " + str(minimisation))
+    d.minimisation = MINIMISATION[minimisation]
     if external not in EXTERNAL:
         raise RuntimeError("unknown external forcing: " + str(external))
     d.external = EXTERNAL[external]

@@ -19,6 +19,10 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <exception>
+#include <map>
+#include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fqsb.h"
@@ -81,7 +85,13 @@ constexpr uint64_t LR_BLOCK = 128;
 
 struct Group {
     fqsb_desc d{};
-    int64_t R = 0, N = 0, rows = 0, cols = 0, nchunk = 0;
+    int64_t R = 0, rows = 0, cols = 0, nchunk = 0;
+    int64_t N = 0;            // largest system of the group (all systems for a uniform batch)
+    int64_t total = 0;        // sum of the system sizes
+    std::vector<int64_t> off; // off[r]: first element of system r in every field; off[R] = total
+    bool uniform = true;      // every system of the same shape and interaction parameters
+    const uint8_t* mask = nullptr; // if set: systems with mask[r] == 0 sit out the next runs
+    int device = 0;
     int pot = 0, inter = 0, ext = 0, minim = 0;
     double *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *f_pot = nullptr, *f_int = nullptr,
            *f_frame = nullptr, *f_damp = nullptr, *f_th = nullptr, *pref = nullptr, *y = nullptr;
@@ -308,8 +318,9 @@ struct Group {
 
     void reset_run(int mode)
     {
-        for (auto& r : hrs) {
-            r.status = fq::ST_RUNNING;
+        for (size_t i = 0; i < hrs.size(); ++i) {
+            auto& r = hrs[i];
+            r.status = (mask && !mask[i]) ? fq::ST_IDLE : fq::ST_RUNNING;
             r.steps = 0;
             r.below = 0;
             r.below2 = 0;
@@ -388,6 +399,9 @@ struct Group {
         }
         if (method == FQSB_LR_MATRIX && pot != FQSB_POT_CUSPY) {
             throw Error("the matrix form of long-range interactions needs the Cuspy potential");
+        }
+        if (method == FQSB_LR_MATRIX && !uniform) {
+            throw Error("the matrix form of long-range interactions needs replicas of one shape");
         }
         if (method == FQSB_LR_MATRIX && !lr_un) {
             const size_t RN = (size_t)(R * N);
@@ -556,7 +570,8 @@ struct Group {
         }
         for (int64_t r = 0; r < R; ++r) {
             const double dp = du[r];
-            const double dfr = dp * (d.k_frame + d.mu) / d.k_frame;
+            const fq::SysView& S = hviews[(size_t)r];
+            const double dfr = dp * (S.k_frame + S.mu) / S.k_frame;
             hrs[r].du_add = dp;
             hrs[r].u_frame += dfr;
             if (du_frame) {
@@ -569,13 +584,15 @@ struct Group {
         }
     }
 
-    // eventDrivenStep detail.h:1933-1960 for every system
+    // eventDrivenStep detail.h:1933-1960 for every system (with mask set:
+    // every system with mask[r] != 0; the others are left as they are, ret 0)
     std::vector<double> event_driven_step(double eps, bool kick, int direction)
     {
         std::vector<double> du(R, 0.0), ret(R, 0.0);
+        auto on = [&](int64_t r) { return !mask || mask[r]; };
         if (kick) {
             for (int64_t r = 0; r < R; ++r) {
-                du[r] = direction > 0 ? eps : -eps;
+                du[r] = on(r) ? (direction > 0 ? eps : -eps) : 0.0;
             }
             advance_uniformly(du, &ret);
             return ret;
@@ -584,7 +601,7 @@ struct Group {
         max_uniform(0, direction, &m);
         bool any = false;
         for (int64_t r = 0; r < R; ++r) {
-            if (m[r] < 0.5 * eps) {
+            if (!on(r) || m[r] < 0.5 * eps) {
                 du[r] = 0.0;
             }
             else {
@@ -595,7 +612,7 @@ struct Group {
         if (any) {
             advance_uniformly(du, &ret);
             for (int64_t r = 0; r < R; ++r) {
-                if (m[r] < 0.5 * eps) {
+                if (!on(r) || m[r] < 0.5 * eps) {
                     ret[r] = 0.0;
                 }
             }
@@ -611,12 +628,9 @@ uint64_t default_stream_state(uint64_t seed, uint64_t& inc)
     return st;
 }
 
-Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride)
+// Checks one description; returns its number of particles.
+int64_t validate_desc(const fqsb_desc& d)
 {
-    if (!dsc) {
-        throw Error("null description");
-    }
-    const fqsb_desc& d = *dsc;
     if (d.rank != 1 && d.rank != 2) {
         throw Error("rank must be 1 or 2");
     }
@@ -639,9 +653,6 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     if (d.nchunk < 8) {
         throw Error("nchunk must be at least 8");
     }
-    if (replicas < 1) {
-        throw Error("at least one replica is needed");
-    }
     const int64_t rows = (int64_t)d.rows, cols = d.rank == 1 ? 1 : (int64_t)d.cols;
     const int64_t N = rows * cols;
     if (N < 1) {
@@ -661,23 +672,61 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     if (d.external == FQSB_EXT_RANDOMFORCING && (!d.dinc_init || !d.dinc)) {
         throw Error("random forcing needs dinc_init and dinc");
     }
+    return N;
+}
+
+// A group of systems that share rank, potential, interactions, external
+// forcing and minimisation (one kernel instantiation), each with its own size
+// and parameters. Fields are stored system after system ([sum N]); the
+// landscape of system r is its [N_r, nchunk_r] block.
+Group* build_group_multi(const std::vector<fqsb_desc>& ds)
+{
+    if (ds.empty()) {
+        throw Error("at least one replica is needed");
+    }
+    const int64_t R = (int64_t)ds.size();
+    std::vector<int64_t> Ns((size_t)R);
+    for (int64_t r = 0; r < R; ++r) {
+        const fqsb_desc& d = ds[(size_t)r];
+        Ns[(size_t)r] = validate_desc(d);
+        const fqsb_desc& d0 = ds[0];
+        if (d.rank != d0.rank || d.potential != d0.potential || d.interactions != d0.interactions ||
+            d.external != d0.external || d.minimisation != d0.minimisation) {
+            throw Error("systems of one group must share rank, potential, interactions, forcing and minimisation");
+        }
+    }
+    const fqsb_desc& d0 = ds[0];
 
     std::unique_ptr<Group> g(new Group());
-    g->d = d;
+    g->d = d0;
     g->d.dinc_init = nullptr;
     g->d.dinc = nullptr;
-    g->R = (int64_t)replicas;
-    g->N = N;
-    g->rows = rows;
-    g->cols = cols;
-    g->nchunk = (int64_t)d.nchunk;
-    g->pot = d.potential;
-    g->inter = inter;
-    g->ext = d.external;
-    g->minim = d.minimisation;
+    g->R = R;
+    g->pot = d0.potential;
+    g->inter = d0.interactions;
+    g->ext = d0.external;
+    g->minim = d0.minimisation;
+    g->off.assign((size_t)R + 1, 0);
+    std::vector<size_t> yoff((size_t)R + 1, 0);
+    bool uniform = true;
+    for (int64_t r = 0; r < R; ++r) {
+        const fqsb_desc& d = ds[(size_t)r];
+        g->off[(size_t)r + 1] = g->off[(size_t)r] + Ns[(size_t)r];
+        yoff[(size_t)r + 1] = yoff[(size_t)r] + (size_t)Ns[(size_t)r] * (size_t)d.nchunk;
+        g->N = std::max<int64_t>(g->N, Ns[(size_t)r]);
+        uniform &= d.rows == d0.rows && d.cols == d0.cols && d.nchunk == d0.nchunk &&
+                   d.k_interactions == d0.k_interactions && d.alpha == d0.alpha;
+    }
+    g->uniform = uniform;
+    g->total = g->off[(size_t)R];
+    g->rows = (int64_t)d0.rows;
+    g->cols = d0.rank == 1 ? 1 : (int64_t)d0.cols;
+    g->nchunk = (int64_t)d0.nchunk;
+    const int inter = g->inter;
 
     int dev = 0;
     HIPC(hipGetDevice(&dev));
+    g->device = dev;
     hipDeviceProp_t prop;
     HIPC(hipGetDeviceProperties(&prop, dev));
     g->lds_limit = prop.sharedMemPerBlock - 2048;
@@ -685,8 +734,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     HIPC(hipEventCreate(&g->ev0));
     HIPC(hipEventCreate(&g->ev1));
 
-    const int64_t R = g->R;
-    const size_t RN = (size_t)(R * N);
+    const size_t RN = (size_t)g->total;
     g->u = dalloc<double>(RN);
     g->v = dalloc<double>(RN);
     g->a = dalloc<double>(RN);
@@ -695,7 +743,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     g->f_int = dalloc<double>(RN);
     g->f_frame = dalloc<double>(RN);
     g->f_damp = dalloc<double>(RN);
-    g->y = dalloc<double>(RN * (size_t)g->nchunk);
+    g->y = dalloc<double>(yoff[(size_t)R]);
     g->start = dalloc<int64_t>(RN);
     g->i_ref = dalloc<int64_t>(RN);
     g->jloc = dalloc<int32_t>(RN);
@@ -708,39 +756,26 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         g->next = dalloc<int64_t>(RN);
         g->dinc = dalloc<int64_t>(RN);
         for (int64_t r = 0; r < R; ++r) {
-            g->h2d(g->next + r * N, d.dinc_init, (size_t)N);
-            g->h2d(g->dinc + r * N, d.dinc, (size_t)N);
+            g->h2d(g->next + g->off[(size_t)r], ds[(size_t)r].dinc_init, (size_t)Ns[(size_t)r]);
+            g->h2d(g->dinc + g->off[(size_t)r], ds[(size_t)r].dinc, (size_t)Ns[(size_t)r]);
         }
     }
-    int64_t lr_m = 0;
-    if (inter == FQSB_INT_LONGRANGE1D) { // LongRange1d ctor detail.h:829-844
-        lr_m = (N - N % 2) / 2;
-        g->lr_m = lr_m;
-        std::vector<double> pr((size_t)N, 0.0);
-        for (int64_t dd = 1; dd < N; ++dd) {
-            pr[(size_t)dd] = d.k_interactions / std::pow((double)dd, d.alpha + 1.0);
+    if (inter == FQSB_INT_LONGRANGE1D) { // LongRange1d ctor detail.h:829-844, per system
+        g->pref = dalloc<double>(RN);
+        for (int64_t r = 0; r < R; ++r) {
+            const fqsb_desc& d = ds[(size_t)r];
+            const int64_t N = Ns[(size_t)r];
+            std::vector<double> pr((size_t)N, 0.0);
+            for (int64_t dd = 1; dd < N; ++dd) {
+                pr[(size_t)dd] = d.k_interactions / std::pow((double)dd, d.alpha + 1.0);
+            }
+            g->h2d(g->pref + g->off[(size_t)r], pr.data(), (size_t)N);
         }
-        g->pref = dalloc<double>((size_t)N);
-        g->h2d(g->pref, pr.data(), (size_t)N);
+        g->lr_m = (g->N - g->N % 2) / 2;
     }
 
-    // distribution parameters with the reference's defaults (see fq_common.hpp)
-    double p[3] = {1.0, 0.0, 0.0};
-    if (maxpar == 3) {
-        p[1] = 1.0;
-    }
-    for (int i = 0; i < d.nparameters; ++i) {
-        p[i] = d.parameters[i];
-    }
-    // alignment (own rule standing in for prrng::alignment(2, 30, 6, false))
-    const int64_t n = g->nchunk;
-    int64_t bb = d.align_buffer < 0 ? 0 : d.align_buffer;
-    bb = std::min<int64_t>(bb, (n - 2) / 4);
-    int64_t mm = std::max<int64_t>(d.align_margin, bb);
-    mm = std::min<int64_t>(mm, n - 2 - bb);
-
-    // resident workgroup if the system fits one, else the streaming path
-    // (FQSB_FORCE_STREAMING=1 selects the streaming path for every size)
+    // resident workgroup if the largest system fits one, else the streaming
+    // path (FQSB_FORCE_STREAMING=1 selects the streaming path for every size)
     if (g->minim != FQSB_MIN_OVERDAMPED) {
         const char* force = std::getenv("FQSB_FORCE_STREAMING");
         try {
@@ -765,11 +800,32 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
     g->hviews.resize((size_t)R);
     g->hrs.assign((size_t)R, fq::RunState{});
     for (int64_t r = 0; r < R; ++r) {
+        const fqsb_desc& d = ds[(size_t)r];
+        const int64_t N = Ns[(size_t)r];
+        // distribution parameters with the reference's defaults (see fq_common.hpp)
+        const int maxpar = (d.distribution == FQSB_DIST_POWER || d.distribution == FQSB_DIST_PARETO ||
+                            d.distribution == FQSB_DIST_WEIBULL)
+                               ? 3
+                               : 2;
+        double p[3] = {1.0, 0.0, 0.0};
+        if (maxpar == 3) {
+            p[1] = 1.0;
+        }
+        for (int i = 0; i < d.nparameters; ++i) {
+            p[i] = d.parameters[i];
+        }
+        // alignment (own rule standing in for prrng::alignment(2, 30, 6, false))
+        const int64_t n = (int64_t)d.nchunk;
+        int64_t bb = d.align_buffer < 0 ? 0 : d.align_buffer;
+        bb = std::min<int64_t>(bb, (n - 2) / 4);
+        int64_t mm = std::max<int64_t>(d.align_margin, bb);
+        mm = std::min<int64_t>(mm, n - 2 - bb);
+
         fq::SysView& S = g->hviews[(size_t)r];
         std::memset(&S, 0, sizeof(S));
         S.N = N;
-        S.rows = rows;
-        S.cols = cols;
+        S.rows = (int64_t)d.rows;
+        S.cols = d.rank == 1 ? 1 : (int64_t)d.cols;
         S.pot = g->pot;
         S.inter = inter;
         S.ext = g->ext;
@@ -787,9 +843,9 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         S.k4 = d.k4;
         S.k4_4 = 0.25 * d.k4;
         S.alpha = d.alpha;
-        S.lr_m = lr_m;
-        S.pref = g->pref;
-        const size_t o = (size_t)(r * N);
+        S.lr_m = (N - N % 2) / 2;
+        const size_t o = (size_t)g->off[(size_t)r];
+        S.pref = g->pref ? g->pref + o : nullptr;
         S.u = g->u + o;
         S.v = g->v + o;
         S.a = g->a + o;
@@ -804,7 +860,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         S.dinc = g->dinc ? g->dinc + o : nullptr;
         S.mean = d.mean;
         S.stddev = d.stddev;
-        S.seed = d.seed + (uint64_t)r * seed_stride;
+        S.seed = d.seed;
         S.offset = d.offset;
         if (g->streaming) {
             S.u_b = g->u2 + o;
@@ -816,7 +872,7 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
             S.yr_b = g->yr2 + o;
             S.jl_b = g->jl2 + o;
         }
-        S.ls.y = g->y + o * (size_t)n;
+        S.ls.y = g->y + yoff[(size_t)r];
         S.ls.start = g->start + o;
         S.ls.jloc = g->jloc + o;
         S.ls.st_begin = g->st_begin + o;
@@ -836,15 +892,33 @@ Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride
         rs.prev = std::numeric_limits<double>::infinity();
         rs.status = fq::ST_RUNNING;
         uint64_t rinc = 0;
-        rs.rng_state = default_stream_state(d.seed_forcing + (uint64_t)r * seed_stride, rinc);
+        rs.rng_state = default_stream_state(d.seed_forcing, rinc);
         rs.rng_inc = rinc;
         rs.min_next = std::numeric_limits<int64_t>::max();
     }
     g->h2d(g->views, g->hviews.data(), (size_t)R);
     g->push();
-    HIPC(fq::launch_landscape_init(g->views, (int)R, N, g->stream));
+    HIPC(fq::launch_landscape_init(g->views, (int)R, g->N, g->stream));
     g->refresh();
     return g.release();
+}
+
+// R replicas of one description: replica r uses seed + r * seed_stride (and
+// seed_forcing + r * seed_stride).
+Group* build_group(const fqsb_desc* dsc, uint64_t replicas, uint64_t seed_stride)
+{
+    if (!dsc) {
+        throw Error("null description");
+    }
+    if (replicas < 1) {
+        throw Error("at least one replica is needed");
+    }
+    std::vector<fqsb_desc> ds((size_t)replicas, *dsc);
+    for (uint64_t r = 0; r < replicas; ++r) {
+        ds[(size_t)r].seed = dsc->seed + r * seed_stride;
+        ds[(size_t)r].seed_forcing = dsc->seed_forcing + r * seed_stride;
+    }
+    return build_group_multi(ds);
 }
 
 int64_t checked_p(const Group& g, uint64_t p)
@@ -863,6 +937,59 @@ struct fqsb_system {
 
 struct fqsb_batch {
     std::unique_ptr<Group> g;
+};
+
+// A ragged set: systems of any classes and sizes, held in groups that share
+// a kernel instantiation and a size class (sizes within a factor 2, so one
+// launch shape fits every system of a group). Operations run on all groups
+// at once, one host thread and one HIP stream per group.
+struct fqsb_ragged {
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<std::pair<size_t, int64_t>> where; // system i -> (group, index in group)
+    std::vector<int64_t> sizes;                    // N of system i
+    std::vector<int64_t> uoff;                     // first element of system i in concatenated arrays
+
+    template <class F>
+    void each(F&& fn)
+    {
+        std::vector<std::exception_ptr> err(groups.size());
+        std::vector<std::thread> th;
+        th.reserve(groups.size());
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            th.emplace_back([&, gi] {
+                try {
+                    HIPC(hipSetDevice(groups[gi]->device));
+                    fn(*groups[gi], gi);
+                }
+                catch (...) {
+                    err[gi] = std::current_exception();
+                }
+            });
+        }
+        for (auto& t : th) {
+            t.join();
+        }
+        for (auto& e : err) {
+            if (e) {
+                std::rethrow_exception(e);
+            }
+        }
+    }
+
+    // per-group masks from a per-system mask (nullptr: all systems)
+    std::vector<std::vector<uint8_t>> masks(const uint8_t* active) const
+    {
+        std::vector<std::vector<uint8_t>> m(groups.size());
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            m[gi].assign((size_t)groups[gi]->R, active ? 0 : 1);
+        }
+        if (active) {
+            for (size_t i = 0; i < where.size(); ++i) {
+                m[where[i].first][(size_t)where[i].second] = active[i] ? 1 : 0;
+            }
+        }
+        return m;
+    }
 };
 
 extern "C" {
@@ -1069,12 +1196,15 @@ int fqsb_time_steps_until_event(fqsb_system* s, double tol, uint64_t niter_tol, 
     });
 }
 
-static void minimise_nopassing(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
-                               uint64_t* ret)
+// No-passing minimise (detail.h:1694-1753) on the device (Jacobi sweeps) for
+// every system of the group (with g.mask: the masked-in ones). Returns per
+// system true if the StopList converged.
+static std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter)
 {
-    // detail.h:1694-1753 on the device (Jacobi sweeps)
-    g.hrs[0].qs_first = g.hrs[0].inc;
-    g.hrs[0].qs_last = g.hrs[0].inc;
+    for (auto& r : g.hrs) {
+        r.qs_first = r.inc;
+        r.qs_last = r.inc;
+    }
     g.reset_run(fq::MODE_MINIMISE);
     g.push();
     int ppt = 1;
@@ -1092,20 +1222,35 @@ static void minimise_nopassing(Group& g, double tol, uint64_t niter_tol, uint64_
     ra.tol2 = tol * tol;
     ra.niter_tol = (int64_t)niter_tol;
     uint64_t done = 0;
-    while (done < max_iter) {
+    auto running = [&] {
+        for (const auto& r : g.hrs) {
+            if (r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) {
+                return true;
+            }
+        }
+        return false;
+    };
+    while (done < max_iter && running()) {
         const uint64_t nb = std::min<uint64_t>(MIN_BLOCK, max_iter - done);
         ra.max_steps = (int64_t)nb;
-        if (!fq::launch_nopassing(ppt, threads, sizeof(double) * 4 * threads, g.views, 1, ra, g.stream)) {
+        if (!fq::launch_nopassing(ppt, threads, sizeof(double) * 4 * threads, g.views, (int)g.R, ra, g.stream)) {
             throw Error("no-passing kernel not available");
         }
         HIPC(hipGetLastError());
         done += nb;
         g.pull();
-        if (g.hrs[0].status == fq::ST_CONVERGED) {
-            break;
-        }
     }
-    if (g.hrs[0].status == fq::ST_CONVERGED) {
+    std::vector<bool> conv((size_t)g.R);
+    for (int64_t r = 0; r < g.R; ++r) {
+        conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
+    }
+    return conv;
+}
+
+static void minimise_nopassing(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                               uint64_t* ret)
+{
+    if (nopassing_run(g, tol, niter_tol, max_iter)[0]) {
         *ret = 0;
         return;
     }
@@ -1503,7 +1648,7 @@ int fqsb_batch_get(fqsb_batch* b, int field, double* out)
 {
     return guard([&] {
         Group& g = *b->g;
-        g.d2h(out, g.field(field), (size_t)(g.R * g.N));
+        g.d2h(out, g.field(field), (size_t)g.total);
     });
 }
 
@@ -1511,7 +1656,7 @@ int fqsb_batch_set(fqsb_batch* b, int field, const double* in)
 {
     return guard([&] {
         Group& g = *b->g;
-        const size_t n = (size_t)(g.R * g.N);
+        const size_t n = (size_t)g.total;
         switch (field) {
             case FQSB_U:
                 g.h2d(g.u, in, n);
@@ -1550,7 +1695,7 @@ int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out)
 {
     return guard([&] {
         Group& g = *b->g;
-        const size_t n = (size_t)(g.R * g.N);
+        const size_t n = (size_t)g.total;
         std::vector<int32_t> jl(n);
         g.d2h(out, g.start, n);
         g.d2h(jl.data(), g.jloc, n);
@@ -1563,6 +1708,276 @@ int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out)
 void* fqsb_batch_stream(fqsb_batch* b)
 {
     return (void*)b->g->stream;
+}
+
+// --- ragged set ----------------------------------------------------------------
+
+int fqsb_ragged_create(const fqsb_desc* descs, uint64_t n, fqsb_ragged** out)
+{
+    return guard([&] {
+        *out = nullptr;
+        if (!descs || n < 1) {
+            throw Error("at least one system is needed");
+        }
+        std::unique_ptr<fqsb_ragged> rg(new fqsb_ragged());
+        // key: kernel class and size class (ceil(log2 N))
+        typedef std::tuple<int, int, int, int, int, int> Key;
+        std::map<Key, std::vector<uint64_t>> classes;
+        rg->sizes.resize((size_t)n);
+        rg->uoff.resize((size_t)n + 1, 0);
+        for (uint64_t i = 0; i < n; ++i) {
+            const fqsb_desc& d = descs[i];
+            const int64_t N = validate_desc(d);
+            rg->sizes[(size_t)i] = N;
+            rg->uoff[(size_t)i + 1] = rg->uoff[(size_t)i] + N;
+            int lg = 0;
+            while ((int64_t(1) << lg) < N) {
+                ++lg;
+            }
+            classes[Key(d.rank, d.potential, d.interactions, d.external, d.minimisation, lg)].push_back(i);
+        }
+        rg->where.resize((size_t)n);
+        for (auto& kv : classes) {
+            std::vector<fqsb_desc> ds;
+            for (uint64_t i : kv.second) {
+                rg->where[(size_t)i] = std::make_pair(rg->groups.size(), (int64_t)ds.size());
+                ds.push_back(descs[i]);
+            }
+            rg->groups.emplace_back(build_group_multi(ds));
+        }
+        *out = rg.release();
+    });
+}
+
+int fqsb_ragged_destroy(fqsb_ragged* r)
+{
+    delete r;
+    return 0;
+}
+
+uint64_t fqsb_ragged_systems(const fqsb_ragged* r)
+{
+    return (uint64_t)r->sizes.size();
+}
+
+uint64_t fqsb_ragged_groups(const fqsb_ragged* r)
+{
+    return (uint64_t)r->groups.size();
+}
+
+int fqsb_ragged_sizes(const fqsb_ragged* r, uint64_t* out)
+{
+    for (size_t i = 0; i < r->sizes.size(); ++i) {
+        out[i] = (uint64_t)r->sizes[i];
+    }
+    return 0;
+}
+
+int fqsb_ragged_time_steps(fqsb_ragged* r, uint64_t n)
+{
+    return guard([&] {
+        if (n > 0) {
+            r->each([&](Group& g, size_t) { g.steps(n, false, 0.0, 0, true, false); });
+        }
+    });
+}
+
+int fqsb_ragged_flow_steps(fqsb_ragged* r, uint64_t n, double v_frame)
+{
+    return guard([&] {
+        if (n > 0) {
+            r->each([&](Group& g, size_t) { g.steps(n, true, v_frame, 0, true, false); });
+        }
+    });
+}
+
+int fqsb_ragged_minimise(fqsb_ragged* r, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                         const uint8_t* active, uint64_t* ret)
+{
+    return guard([&] {
+        auto m = r->masks(active);
+        std::vector<std::vector<uint64_t>> gret(r->groups.size());
+        for (auto& g : r->groups) {
+            if (g->minim == FQSB_MIN_NONE) {
+                for (size_t i = 0; i < r->where.size(); ++i) {
+                    if (r->groups[r->where[i].first].get() == g.get() && (!active || active[i])) {
+                        throw Error("Minimisation not implementated");
+                    }
+                }
+            }
+        }
+        r->each([&](Group& g, size_t gi) {
+            gret[gi].assign((size_t)g.R, 0);
+            bool any = false;
+            for (uint8_t x : m[gi]) {
+                any |= x != 0;
+            }
+            if (!any) {
+                return;
+            }
+            g.mask = m[gi].data();
+            try {
+                if (g.minim == FQSB_MIN_OVERDAMPED) {
+                    auto conv = nopassing_run(g, tol, niter_tol, max_iter);
+                    for (int64_t k = 0; k < g.R; ++k) {
+                        gret[gi][(size_t)k] = (!m[gi][(size_t)k] || conv[(size_t)k]) ? 0 : max_iter + 1;
+                    }
+                }
+                else {
+                    fq::RunArgs ra{};
+                    ra.mode = fq::MODE_MINIMISE;
+                    ra.tol = tol;
+                    ra.tol2 = tol * tol;
+                    ra.niter_tol = (int64_t)niter_tol;
+                    g.run(ra, max_iter, MIN_BLOCK, false);
+                    for (int64_t k = 0; k < g.R; ++k) {
+                        const auto& st = g.hrs[(size_t)k];
+                        if (st.status == fq::ST_NAN) {
+                            throw Error("NaN entries found");
+                        }
+                        gret[gi][(size_t)k] = (st.status == fq::ST_IDLE || st.status == fq::ST_CONVERGED)
+                                                  ? 0
+                                                  : max_iter + 1;
+                    }
+                }
+            }
+            catch (...) {
+                g.mask = nullptr;
+                throw;
+            }
+            g.mask = nullptr;
+        });
+        for (size_t i = 0; i < r->where.size(); ++i) {
+            ret[i] = gret[r->where[i].first][(size_t)r->where[i].second];
+            if (ret[i] != 0 && max_iter_is_error) {
+                throw Error("No convergence found");
+            }
+        }
+    });
+}
+
+int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int direction, const uint8_t* active,
+                                  double* ret)
+{
+    return guard([&] {
+        auto m = r->masks(active);
+        std::vector<std::vector<double>> gret(r->groups.size());
+        for (size_t gi = 0; gi < r->groups.size(); ++gi) {
+            const Group& g = *r->groups[gi];
+            bool any = false;
+            for (uint8_t x : m[gi]) {
+                any |= x != 0;
+            }
+            if (any && g.ext) {
+                throw Error("eventDrivenStep is not available for this system");
+            }
+            if (any && !kick && g.pot == FQSB_POT_SMOOTH) {
+                throw Error("Operation not possible.");
+            }
+        }
+        r->each([&](Group& g, size_t gi) {
+            gret[gi].assign((size_t)g.R, 0.0);
+            bool any = false;
+            for (uint8_t x : m[gi]) {
+                any |= x != 0;
+            }
+            if (!any) {
+                return;
+            }
+            g.mask = m[gi].data();
+            try {
+                gret[gi] = g.event_driven_step(eps, kick != 0, direction);
+            }
+            catch (...) {
+                g.mask = nullptr;
+                throw;
+            }
+            g.mask = nullptr;
+        });
+        for (size_t i = 0; i < r->where.size(); ++i) {
+            ret[i] = gret[r->where[i].first][(size_t)r->where[i].second];
+        }
+    });
+}
+
+int fqsb_ragged_get(fqsb_ragged* r, int field, double* out)
+{
+    return guard([&] {
+        for (size_t gi = 0; gi < r->groups.size(); ++gi) {
+            Group& g = *r->groups[gi];
+            std::vector<double> h((size_t)g.total);
+            g.d2h(h.data(), g.field(field), h.size());
+            for (size_t i = 0; i < r->where.size(); ++i) {
+                if (r->where[i].first == gi) {
+                    const int64_t k = r->where[i].second;
+                    std::copy(h.begin() + g.off[(size_t)k], h.begin() + g.off[(size_t)k + 1], out + r->uoff[i]);
+                }
+            }
+        }
+    });
+}
+
+int fqsb_ragged_set(fqsb_ragged* r, int field, const double* in)
+{
+    return guard([&] {
+        if (field != FQSB_U && field != FQSB_V && field != FQSB_A) {
+            throw Error("field is read-only");
+        }
+        for (size_t gi = 0; gi < r->groups.size(); ++gi) {
+            Group& g = *r->groups[gi];
+            std::vector<double> h((size_t)g.total);
+            for (size_t i = 0; i < r->where.size(); ++i) {
+                if (r->where[i].first == gi) {
+                    const int64_t k = r->where[i].second;
+                    std::copy(in + r->uoff[i], in + r->uoff[i + 1], h.begin() + g.off[(size_t)k]);
+                }
+            }
+            g.h2d(g.field(field), h.data(), h.size());
+            if (field == FQSB_U) {
+                g.updated_u();
+            }
+            else if (field == FQSB_V) {
+                g.updated_v();
+            }
+        }
+    });
+}
+
+int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out)
+{
+    for (size_t i = 0; i < r->where.size(); ++i) {
+        out[i] = r->groups[r->where[i].first]->hrs[(size_t)r->where[i].second].u_frame;
+    }
+    return 0;
+}
+
+int fqsb_ragged_get_inc(fqsb_ragged* r, int64_t* out)
+{
+    for (size_t i = 0; i < r->where.size(); ++i) {
+        out[i] = r->groups[r->where[i].first]->hrs[(size_t)r->where[i].second].inc;
+    }
+    return 0;
+}
+
+int fqsb_ragged_chunk_index_at_align(fqsb_ragged* r, int64_t* out)
+{
+    return guard([&] {
+        for (size_t gi = 0; gi < r->groups.size(); ++gi) {
+            Group& g = *r->groups[gi];
+            std::vector<int64_t> st((size_t)g.total);
+            std::vector<int32_t> jl((size_t)g.total);
+            g.d2h(st.data(), g.start, st.size());
+            g.d2h(jl.data(), g.jloc, jl.size());
+            for (size_t i = 0; i < r->where.size(); ++i) {
+                if (r->where[i].first == gi) {
+                    const int64_t k = r->where[i].second;
+                    for (int64_t p = g.off[(size_t)k]; p < g.off[(size_t)k + 1]; ++p) {
+                        out[r->uoff[i] + (p - g.off[(size_t)k])] = st[(size_t)p] + jl[(size_t)p];
+                    }
+                }
+            }
+        }
+    });
 }
 
 } // extern "C"

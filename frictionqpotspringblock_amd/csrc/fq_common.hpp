@@ -55,7 +55,8 @@ enum Status : int32_t {
     ST_TRUNCATED = 3, // A_truncate / S_truncate reached
     ST_NAN = 4,       // NaN in u
     ST_BUDGET = 5,    // step budget of this launch used up
-    ST_REFILL = 6     // the next step needs a landscape refill (k_refill, then relaunch)
+    ST_REFILL = 6,    // the next step needs a landscape refill (k_refill, then relaunch)
+    ST_IDLE = 7       // left out of this run by the caller
 };
 
 // Components recomputed by k_forces (fq_service.hpp).

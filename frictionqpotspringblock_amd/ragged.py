@@ -1,0 +1,159 @@
+"""Ragged sets: systems of any classes, sizes and parameters advanced together.
+
+No reference counterpart (the reference advances one system per object);
+every operation applies to each system exactly as the single-system call of
+the reference would (System::timeSteps, minimise, eventDrivenStep,
+detail.h:1577-1960), so the results are bit-identical to separate systems.
+Systems that share a kernel class and a size class run in one launch, and the
+launches of different classes run concurrently.
+
+Thin ctypes layer over the ``fqsb_ragged_*`` functions of ``libfqsb.so``
+(include/fqsb.h).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .batch import FIELD, _check, _dp, _ip, _up, capi, make_desc
+
+_u8p = C.POINTER(C.c_uint8)
+
+_CLASSES = {
+    # class name (Line1d / Line2d of the reference) -> make_desc arguments
+    "System_Cuspy_Laplace": dict(potential="Cuspy", interactions="Laplace1d"),
+    "System_Cuspy_Laplace_Nopassing": dict(potential="Cuspy", interactions="Laplace1d", minimisation="Overdamped"),
+    "System_SemiSmooth_Laplace": dict(potential="SemiSmooth", interactions="Laplace1d"),
+    "System_Smooth_Laplace": dict(potential="Smooth", interactions="Laplace1d"),
+    "System_Cuspy_Quartic": dict(potential="Cuspy", interactions="Quartic1d"),
+    "System_SemiSmooth_Quartic": dict(potential="SemiSmooth", interactions="Quartic1d"),
+    "System_Smooth_Quartic": dict(potential="Smooth", interactions="Quartic1d"),
+    "System_Cuspy_QuarticGradient": dict(potential="Cuspy", interactions="QuarticGradient1d"),
+    "System_Cuspy_LongRange": dict(potential="Cuspy", interactions="LongRange1d"),
+    "Line2d.System_Cuspy_Laplace": dict(potential="Cuspy", interactions="Laplace2d"),
+    "Line2d.System_Cuspy_QuarticGradient": dict(potential="Cuspy", interactions="QuarticGradient2d"),
+}
+
+
+class Ragged:
+    """A set of systems, each given as ``(class_name, kwargs)`` with the
+    reference's constructor arguments (``Line1d.System_*``; ``Line2d.`` prefix
+    for the 2-d classes), e.g. ``("System_Smooth_Laplace", dict(m=1, ...))``."""
+
+    def __init__(self, systems):
+        self._descs = []
+        self._shapes = []
+        for name, kw in systems:
+            if name not in _CLASSES:
+                raise RuntimeError("unknown system class: " + str(name))
+            self._descs.append(make_desc(**_CLASSES[name], **kw))
+            self._shapes.append([int(s) for s in kw["shape"]])
+        n = len(self._descs)
+        arr = (type(self._descs[0]) * n)(*self._descs)
+        self._arr = arr
+        h = C.c_void_p()
+        _check(capi().fqsb_ragged_create(arr, n, C.byref(h)))
+        self._h = h
+        self._n = n
+        self._sizes = [int(np.prod(s)) for s in self._shapes]
+        self._offsets = np.concatenate([[0], np.cumsum(self._sizes)]).astype(int)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None:
+            try:
+                capi().fqsb_ragged_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __len__(self):
+        return self._n
+
+    @property
+    def sizes(self):
+        return list(self._sizes)
+
+    @property
+    def groups(self):
+        """Number of launch groups (kernel class x size class)."""
+        return int(capi().fqsb_ragged_groups(self._h))
+
+    def _mask(self, active):
+        if active is None:
+            return None
+        m = np.ascontiguousarray(np.asarray(active, dtype=bool).astype(np.uint8))
+        if m.size != self._n:
+            raise RuntimeError("active must have one entry per system")
+        return m
+
+    def timeSteps(self, n):
+        _check(capi().fqsb_ragged_time_steps(self._h, int(n)))
+
+    def flowSteps(self, n, v_frame):
+        _check(capi().fqsb_ragged_flow_steps(self._h, int(n), float(v_frame)))
+
+    def minimise(self, tol=1e-5, niter_tol=10, max_iter=int(1e9), max_iter_is_error=True, active=None):
+        ret = np.zeros(self._n, dtype=np.uint64)
+        m = self._mask(active)
+        _check(
+            capi().fqsb_ragged_minimise(
+                self._h, float(tol), int(niter_tol), int(max_iter), int(bool(max_iter_is_error)),
+                None if m is None else m.ctypes.data_as(_u8p), ret.ctypes.data_as(_up),
+            )
+        )
+        return ret
+
+    def eventDrivenStep(self, eps, kick, direction=1, active=None):
+        ret = np.zeros(self._n)
+        m = self._mask(active)
+        _check(
+            capi().fqsb_ragged_event_driven_step(
+                self._h, float(eps), int(bool(kick)), int(direction),
+                None if m is None else m.ctypes.data_as(_u8p), ret.ctypes.data_as(_dp),
+            )
+        )
+        return ret
+
+    def _split(self, flat):
+        return [flat[a:b].reshape(s) for a, b, s in zip(self._offsets[:-1], self._offsets[1:], self._shapes)]
+
+    def _get(self, name):
+        out = np.empty(self._offsets[-1])
+        _check(capi().fqsb_ragged_get(self._h, FIELD[name], out.ctypes.data_as(_dp)))
+        return self._split(out)
+
+    def _set(self, name, values):
+        flat = np.ascontiguousarray(np.concatenate([np.ravel(np.asarray(v, dtype=float)) for v in values]))
+        if flat.size != self._offsets[-1]:
+            raise RuntimeError("one array per system of the system's size is needed")
+        _check(capi().fqsb_ragged_set(self._h, FIELD[name], flat.ctypes.data_as(_dp)))
+
+    u = property(lambda self: self._get("u"), lambda self, v: self._set("u", v))
+    v = property(lambda self: self._get("v"), lambda self, v: self._set("v", v))
+    a = property(lambda self: self._get("a"), lambda self, v: self._set("a", v))
+    f = property(lambda self: self._get("f"))
+    f_potential = property(lambda self: self._get("f_potential"))
+    f_interactions = property(lambda self: self._get("f_interactions"))
+    f_frame = property(lambda self: self._get("f_frame"))
+    f_damping = property(lambda self: self._get("f_damping"))
+
+    @property
+    def u_frame(self):
+        out = np.empty(self._n)
+        _check(capi().fqsb_ragged_get_u_frame(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    @property
+    def inc(self):
+        out = np.empty(self._n, dtype=np.int64)
+        _check(capi().fqsb_ragged_get_inc(self._h, out.ctypes.data_as(_ip)))
+        return out
+
+    @property
+    def index_at_align(self):
+        out = np.empty(self._offsets[-1], dtype=np.int64)
+        _check(capi().fqsb_ragged_chunk_index_at_align(self._h, out.ctypes.data_as(_ip)))
+        return self._split(out)

@@ -244,6 +244,43 @@ int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out);
 /* The HIP stream (hipStream_t) the batch launches on. */
 void* fqsb_batch_stream(fqsb_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Ragged set (no reference counterpart): n systems of any classes, sizes and
+ * parameters (descs[i] as for fqsb_create), advanced together. Systems that
+ * share a kernel class and a size class (sizes within a factor 2) run in one
+ * launch; the groups run concurrently. Every operation applies to each
+ * system exactly as the corresponding single-system call would, so results
+ * are bit-identical to n separate systems. Arrays are the systems' fields
+ * concatenated in the order of descs (sum of the sizes); scalars [n].
+ * `active` (NULL: all) selects the systems an operation applies to; the
+ * others are left unchanged and get ret 0.
+ * ---------------------------------------------------------------------- */
+typedef struct fqsb_ragged fqsb_ragged;
+
+int fqsb_ragged_create(const fqsb_desc* descs, uint64_t n, fqsb_ragged** out);
+int fqsb_ragged_destroy(fqsb_ragged* r);
+uint64_t fqsb_ragged_systems(const fqsb_ragged* r);
+uint64_t fqsb_ragged_groups(const fqsb_ragged* r);
+int fqsb_ragged_sizes(const fqsb_ragged* r, uint64_t* out);
+/* timeSteps / flowSteps (detail.h:1577-1583, 1637-1645) on every system. */
+int fqsb_ragged_time_steps(fqsb_ragged* r, uint64_t n);
+int fqsb_ragged_flow_steps(fqsb_ragged* r, uint64_t n, double v_frame);
+/* minimise (detail.h:1676-1792; no-passing systems: detail.h:1694-1753);
+ * ret[i] as System::minimise. Fails ("Minimisation not implementated") if an
+ * active system has no minimisation. */
+int fqsb_ragged_minimise(fqsb_ragged* r, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                         const uint8_t* active, uint64_t* ret);
+/* eventDrivenStep (detail.h:1933-1960); fails with "Operation not possible."
+ * if kick == 0 and an active system has the Smooth potential (its
+ * maxUniformDisplacement raises, detail.h:414-422). */
+int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int direction, const uint8_t* active,
+                                  double* ret);
+int fqsb_ragged_get(fqsb_ragged* r, int field, double* out);
+int fqsb_ragged_set(fqsb_ragged* r, int field, const double* in);
+int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out);
+int fqsb_ragged_get_inc(fqsb_ragged* r, int64_t* out);
+int fqsb_ragged_chunk_index_at_align(fqsb_ragged* r, int64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
