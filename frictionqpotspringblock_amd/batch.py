@@ -163,6 +163,15 @@ SIGNATURES = {
     "fqsb_batch_get_inc": (C.c_int, [_P, _ip]),
     "fqsb_batch_chunk_index_at_align": (C.c_int, [_P, _ip]),
     "fqsb_batch_stream": (C.c_void_p, [_P]),
+    "fqsb_batch_set_parameters": (C.c_int, [_P, _dp]),
+    "fqsb_batch_get_parameters": (C.c_int, [_P, _dp]),
+    "fqsb_batch_nchunk": (C.c_uint64, [_P]),
+    "fqsb_batch_get_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "fqsb_batch_set_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "fqsb_batch_chunk_data_device": (C.c_int, [_P, C.c_void_p]),
+    "fqsb_batch_set_chunk_data_device": (C.c_int, [_P, C.c_void_p]),
+    "fqsb_batch_adjoint_forward": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int, C.c_double]),
+    "fqsb_batch_adjoint_backward_device": (C.c_int, [_P] + [C.c_void_p] * 8),
     "fqsb_ragged_create": (C.c_int, [C.POINTER(Desc), C.c_uint64, C.POINTER(_P)]),
     "fqsb_ragged_destroy": (C.c_int, [_P]),
     "fqsb_ragged_systems": (C.c_uint64, [_P]),

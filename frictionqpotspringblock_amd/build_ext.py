@@ -46,6 +46,7 @@ HIP_SOURCES = [
     "fq_service.hip",
     "fq_stream.hip",
     "fq_longrange.hip",
+    "fq_adjoint.hip",
     "fq_res_cuspy.hip",
     "fq_res_semismooth.hip",
     "fq_res_smooth.hip",

@@ -1,0 +1,339 @@
+// Discrete adjoint of System::timeStep (detail.h:1539-1570) for Cuspy +
+// Laplace1d replicas (no reference counterpart). With the well index of every
+// particle taken as a constant at every step (the indices the trajectory
+// visited), one step is the map (u, v, a) -> (u1, v3, a3):
+//   u1 = u + dt v + dt^2/2 a
+//   g  = k_frame (u_frame - u1) + mu (ym - u1) + k L u1     ym = (y_l + y_r) / 2
+//   v1 = v + dt a,               a1 = (g - eta v1) / m
+//   v2 = v + dt/2 (a + a1),      a2 = (g - eta v2) / m
+//   v3 = v + dt/2 (a + a2),      a3 = (g - eta v3) / m
+// (L: periodic second difference). The adjoint runs the step backwards for the
+// cotangents of (u1, v3, a3) and accumulates the derivatives with respect to
+// mu, k, k_frame, eta and the yield positions y_l, y_r of the visited wells.
+//
+// A run of n steps is split into windows of K steps. For each window, last
+// to first: k_adj_record re-runs the window from its checkpoint (the same
+// expressions in the same order as the resident kernel, so the same
+// trajectory and the same wells) and writes a tape of the state at the start
+// of every step and the well used by it; k_adj_reverse then walks the tape
+// backwards. One workgroup per replica, PPT consecutive particles per thread,
+// neighbours exchanged through LDS.
+#pragma once
+
+#include "fq_physics.hpp"
+#include "fq_resident.hpp"
+
+namespace fq {
+
+
+// Record one window of `steps` steps from the work state.
+template <int PPT, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__ views, AdjBuffers ab, int steps,
+                                                     int flow, double v_frame)
+{
+    __shared__ double h_first[2][MAXT];
+    __shared__ double h_last[2][MAXT];
+    const int r = blockIdx.x;
+    const SysView& S = views[r];
+    const int t = threadIdx.x;
+    const int64_t N = ab.N;
+    const int64_t off = (int64_t)r * N;
+    const int64_t RN = ab.R * N;
+    const int64_t base = (int64_t)t * PPT;
+    const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
+    const int Tact = (int)((N + PPT - 1) / PPT);
+    const Landscape& L = S.ls;
+    const double* const Y = L.y;
+    const int64_t nch = L.nchunk;
+    const int32_t jmin = (int32_t)L.buffer;
+    const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
+
+    // the yield positions of the current wells are re-read from the chunk
+    // (L1/L2) where needed, which keeps the registers to u, v, a, well index
+    double u[PPT], v[PPT], a[PPT];
+    int32_t jl[PPT];
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        u[i] = v[i] = a[i] = 0.0;
+        jl[i] = 0;
+        if (i < cnt) {
+            const int64_t p = base + i;
+            u[i] = ab.wu[off + p];
+            v[i] = ab.wv[off + p];
+            a[i] = ab.wa[off + p];
+            jl[i] = ab.wj[off + p];
+        }
+    }
+    const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, inv_m = S.inv_m, kf = S.k_frame;
+    const double dv = v_frame * dt;
+    double uf = ab.wuf[r];
+    int bad = 0;
+
+    for (int n = 0; n < steps; ++n) {
+        const int b = n & 1;
+        if (flow) {
+            uf = uf + dv;
+        }
+        const int64_t tn = (int64_t)n * RN + off;
+        if (t == 0) {
+            ab.tuf[(int64_t)n * ab.R + r] = uf;
+        }
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t p = base + i;
+                ab.tu[tn + p] = u[i];
+                ab.tv[tn + p] = v[i];
+                ab.ta[tn + p] = a[i];
+                u[i] = u[i] + dt * v[i] + c2 * a[i];
+                const double x = u[i];
+                const double* row = Y + p * nch;
+                int32_t j = jl[i];
+                double lo = row[j], hi = row[j + 1];
+                if (!(lo <= x && x < hi)) {
+                    while (x >= hi && j < jmax) {
+                        ++j;
+                        lo = hi;
+                        hi = row[j + 1];
+                    }
+                    while (x < lo && j > jmin) {
+                        --j;
+                        hi = lo;
+                        lo = row[j];
+                    }
+                    bad |= !(lo <= x && x < hi);
+                    jl[i] = j;
+                }
+                ab.tj[tn + p] = jl[i];
+            }
+        }
+        if (cnt > 0) {
+            h_first[b][t] = u[0];
+            h_last[b][t] = u[cnt - 1];
+        }
+        __syncthreads();
+        const double left = h_last[b][t == 0 ? Tact - 1 : t - 1];
+        const double right = h_first[b][t == Tact - 1 ? 0 : t + 1];
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const double um = i == 0 ? left : u[i > 0 ? i - 1 : 0];
+                const double up = i == cnt - 1 ? right : u[i + 1 < PPT ? i + 1 : i];
+                const double* row = Y + (base + i) * nch;
+                const double fi = chain_force<INT_LAPLACE1D>(S, um, u[i], up);
+                const double fp = potential_force<POT_CUSPY>(S, u[i], row[jl[i]], row[jl[i] + 1]);
+                const double ff = kf * (uf - u[i]);
+                const double sum3 = ff + fp + fi;
+                const double vn = v[i], an = a[i];
+                double vv = vn + dt * an;
+                double f = sum3 + meta * vv;
+                double aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                v[i] = vv;
+                a[i] = aa;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        if (i < cnt) {
+            const int64_t p = base + i;
+            ab.wu[off + p] = u[i];
+            ab.wv[off + p] = v[i];
+            ab.wa[off + p] = a[i];
+            ab.wj[off + p] = jl[i];
+        }
+    }
+    if (bad) {
+        *ab.err = 1;
+    }
+}
+
+// Walk the tape of one window backwards, updating the adjoint state and the
+// parameter / yield-position derivatives. Since dg/dy_l = dg/dy_r = mu/2, one
+// accumulator per particle collects the derivative for both yield positions
+// of its current well; it is added to the output when the well changes.
+template <int PPT, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict__ views, AdjBuffers ab, int steps)
+{
+    // neighbour exchange: slips u1 (h_*) and cotangents of g (g_*) of a step;
+    // two barriers per step (a buffer is rewritten only after the barrier
+    // that follows every read of it)
+    __shared__ double h_first[MAXT];
+    __shared__ double h_last[MAXT];
+    __shared__ double g_first[MAXT];
+    __shared__ double g_last[MAXT];
+    __shared__ double s_red[4][MAXT / 64];
+    const int r = blockIdx.x;
+    const SysView& S = views[r];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+    const int64_t N = ab.N;
+    const int64_t off = (int64_t)r * N;
+    const int64_t RN = ab.R * N;
+    const int64_t base = (int64_t)t * PPT;
+    const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
+    const int Tact = (int)((N + PPT - 1) / PPT);
+    const double* const Y = S.ls.y;
+    const int64_t nch = S.ls.nchunk;
+
+    const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, w = S.inv_m, kf = S.k_frame;
+    const double mu = S.mu, k = S.k;
+    const double mweta = meta * w; // -eta / m
+    const double hmu = 0.5 * mu;
+
+    // U, V, A: cotangents of u, v, a after the step being reversed
+    double U[PPT], V[PPT], A[PPT], acc[PPT];
+    int32_t jc[PPT];
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        U[i] = V[i] = A[i] = acc[i] = 0.0;
+        jc[i] = -1;
+        if (i < cnt) {
+            const int64_t p = base + i;
+            U[i] = ab.au[off + p];
+            V[i] = ab.av[off + p];
+            A[i] = ab.aa[off + p];
+        }
+    }
+    double gmu = 0.0, gk = 0.0, gkf = 0.0, geta = 0.0;
+
+    auto flush = [&](int i) {
+        if (ab.gy && jc[i] >= 0) {
+            double* row = ab.gy + (off + base + i) * nch;
+            row[jc[i]] += acc[i];
+            row[jc[i] + 1] += acc[i];
+        }
+        acc[i] = 0.0;
+    };
+
+    for (int n = steps - 1; n >= 0; --n) {
+        const int64_t tn = (int64_t)n * RN + off;
+        const double uf = ab.tuf[(int64_t)n * ab.R + r];
+        double x[PPT]; // u1 of the step, then the cotangent of g
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            x[i] = 0.0;
+            if (i < cnt) {
+                const int64_t p = base + i;
+                x[i] = ab.tu[tn + p] + dt * ab.tv[tn + p] + c2 * ab.ta[tn + p];
+            }
+        }
+        if (cnt > 0) {
+            h_first[t] = x[0];
+            h_last[t] = x[cnt - 1];
+        }
+        __syncthreads();
+        const double left = h_last[t == 0 ? Tact - 1 : t - 1];
+        const double right = h_first[t == Tact - 1 ? 0 : t + 1];
+        double xm = left; // u1 of the particle before i (its original value)
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t p = base + i;
+                const double u1 = x[i];
+                const double up = i == cnt - 1 ? right : x[i + 1 < PPT ? i + 1 : i];
+                const double lu = xm - 2 * u1 + up;
+                xm = u1;
+                const int32_t j = ab.tj[tn + p];
+                if (j != jc[i]) {
+                    flush(i);
+                    jc[i] = j;
+                }
+                const double ym = 0.5 * (Y[p * nch + j] + Y[p * nch + j + 1]);
+                const double v = ab.tv[tn + p], a = ab.ta[tn + p];
+                // forward quantities of the step
+                const double g = kf * (uf - u1) + (ym - u1) * mu + lu * k;
+                const double v1 = v + dt * a;
+                const double a1 = (g + meta * v1) * w;
+                const double v2 = v + hdt * (a + a1);
+                const double a2 = (g + meta * v2) * w;
+                const double v3 = v + hdt * (a + a2);
+                // a3 = w (g - eta v3)
+                double G = w * A[i];
+                const double V3 = V[i] + mweta * A[i];
+                geta -= w * v3 * A[i];
+                // v3 = v + hdt a + hdt a2
+                double vb = V3, ab_ = hdt * V3;
+                const double A2 = hdt * V3;
+                G += w * A2;
+                const double V2 = mweta * A2;
+                geta -= w * v2 * A2;
+                // v2 = v + hdt a + hdt a1
+                vb += V2;
+                ab_ += hdt * V2;
+                const double A1 = hdt * V2;
+                G += w * A1;
+                const double V1 = mweta * A1;
+                geta -= w * v1 * A1;
+                // v1 = v + dt a
+                vb += V1;
+                ab_ += dt * V1;
+                gkf += (uf - u1) * G;
+                gmu += (ym - u1) * G;
+                gk += lu * G;
+                acc[i] += hmu * G;
+                V[i] = vb;  // partial cotangents of v, a (the u1 part follows)
+                A[i] = ab_;
+                x[i] = G;
+            }
+        }
+        if (cnt > 0) {
+            g_first[t] = x[0];
+            g_last[t] = x[cnt - 1];
+        }
+        __syncthreads();
+        const double gleft = g_last[t == 0 ? Tact - 1 : t - 1];
+        const double gright = g_first[t == Tact - 1 ? 0 : t + 1];
+        double gm = gleft;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const double G = x[i];
+                const double gp = i == cnt - 1 ? gright : x[i + 1 < PPT ? i + 1 : i];
+                // g depends on u1 through -(k_frame + mu) u1 + k L u1 (L symmetric)
+                const double U1 = U[i] - (kf + mu) * G + k * (gm - 2 * G + gp);
+                gm = G;
+                // u1 = u + dt v + c2 a
+                U[i] = U1;
+                V[i] = V[i] + dt * U1;
+                A[i] = A[i] + c2 * U1;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        if (i < cnt) {
+            const int64_t p = base + i;
+            ab.au[off + p] = U[i];
+            ab.av[off + p] = V[i];
+            ab.aa[off + p] = A[i];
+            flush(i);
+        }
+    }
+    // parameter derivatives: block sums in a fixed order, added to the totals
+    gmu = wave_sum(gmu);
+    gk = wave_sum(gk);
+    gkf = wave_sum(gkf);
+    geta = wave_sum(geta);
+    if (lane == 0) {
+        s_red[0][wave] = gmu;
+        s_red[1][wave] = gk;
+        s_red[2][wave] = gkf;
+        s_red[3][wave] = geta;
+    }
+    __syncthreads();
+    if (t < 4) {
+        double s = 0.0;
+        for (int q = 0; q < nw; ++q) {
+            s += s_red[t][q];
+        }
+        ab.gpar[(int64_t)r * 4 + t] += s;
+    }
+}
+
+} // namespace fq

@@ -92,6 +92,23 @@ struct Group {
     bool uniform = true;      // every system of the same shape and interaction parameters
     const uint8_t* mask = nullptr; // if set: systems with mask[r] == 0 sit out the next runs
     int device = 0;
+    int64_t refills = 0; // landscape refills so far (the adjoint needs runs without)
+    // adjoint (fq_adjoint.hpp): checkpoints of the last adjoint_forward, tape
+    // of one window, work and adjoint state
+    int64_t adj_n = 0, adj_K = 0, adj_nw = 0;
+    int adj_flow = 0;
+    double adj_vf = 0.0;
+    bool adj_valid = false;
+    double *ck_u = nullptr, *ck_v = nullptr, *ck_a = nullptr;
+    int32_t* ck_j = nullptr;
+    std::vector<double> ck_uf; // [nw][R]
+    double *tp_u = nullptr, *tp_v = nullptr, *tp_a = nullptr, *tp_uf = nullptr;
+    int32_t* tp_j = nullptr;
+    double *wk_u = nullptr, *wk_v = nullptr, *wk_a = nullptr, *wk_uf = nullptr;
+    int32_t* wk_j = nullptr;
+    double *ad_u = nullptr, *ad_v = nullptr, *ad_a = nullptr, *ad_par = nullptr;
+    int32_t* ad_err = nullptr;
+    int64_t ck_cap = 0, tp_cap = 0; // windows / steps the buffers hold
     int pot = 0, inter = 0, ext = 0, minim = 0;
     double *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *f_pot = nullptr, *f_int = nullptr,
            *f_frame = nullptr, *f_damp = nullptr, *f_th = nullptr, *pref = nullptr, *y = nullptr;
@@ -157,6 +174,7 @@ struct Group {
         dfree(jl2);
         dfree(lr_un);
         dfree(lr_part);
+        free_adjoint();
         dfree(lr_yl);
         dfree(lr_yr);
         if (ev0) {
@@ -368,6 +386,7 @@ struct Group {
             }
             if (refill) {
                 HIPC(fq::launch_refill(views, (int)R, stream));
+                ++refills;
                 for (auto& r : hrs) {
                     if (r.status == fq::ST_REFILL) {
                         r.status = fq::ST_RUNNING;
@@ -387,6 +406,226 @@ struct Group {
         }
         pull();
         return ms;
+    }
+
+
+    void free_adjoint()
+    {
+        for (double** q : {&ck_u, &ck_v, &ck_a, &tp_u, &tp_v, &tp_a, &tp_uf, &wk_u, &wk_v, &wk_a, &wk_uf, &ad_u,
+                           &ad_v, &ad_a, &ad_par}) {
+            dfree(*q);
+            *q = nullptr;
+        }
+        for (int32_t** q : {&ck_j, &tp_j, &wk_j, &ad_err}) {
+            dfree(*q);
+            *q = nullptr;
+        }
+        ck_cap = tp_cap = 0;
+        adj_valid = false;
+    }
+
+    fq::AdjBuffers adj_buffers(double* gy) const
+    {
+        fq::AdjBuffers ab{};
+        ab.tu = tp_u;
+        ab.tv = tp_v;
+        ab.ta = tp_a;
+        ab.tj = tp_j;
+        ab.tuf = tp_uf;
+        ab.wu = wk_u;
+        ab.wv = wk_v;
+        ab.wa = wk_a;
+        ab.wj = wk_j;
+        ab.wuf = wk_uf;
+        ab.au = ad_u;
+        ab.av = ad_v;
+        ab.aa = ad_a;
+        ab.gpar = ad_par;
+        ab.gy = gy;
+        ab.err = ad_err;
+        ab.N = N;
+        ab.R = R;
+        return ab;
+    }
+
+    void adjoint_check() const
+    {
+        if (pot != FQSB_POT_CUSPY || inter != FQSB_INT_LAPLACE1D || ext != FQSB_EXT_NONE ||
+            minim == FQSB_MIN_OVERDAMPED || !uniform) {
+            throw Error("the adjoint needs replicas of one shape with the Cuspy potential, Laplace1d "
+                        "interactions and no external forcing");
+        }
+        if (N > 4096 || N < 3) {
+            throw Error("the adjoint supports chains of 3 to 4096 particles");
+        }
+    }
+
+    // n steps (timeSteps, or flowSteps at v_frame), keeping a checkpoint every K
+    void adjoint_forward(uint64_t n, uint64_t K, bool flow, double v_frame)
+    {
+        adjoint_check();
+        if (K < 1 || n < 1) {
+            throw Error("the adjoint needs n >= 1 steps and windows of K >= 1 steps");
+        }
+        adj_valid = false;
+        const int64_t nw = (int64_t)((n + K - 1) / K);
+        const size_t RN = (size_t)total;
+        if (nw > ck_cap) {
+            dfree(ck_u);
+            dfree(ck_v);
+            dfree(ck_a);
+            dfree(ck_j);
+            ck_u = dalloc<double>(RN * (size_t)nw);
+            ck_v = dalloc<double>(RN * (size_t)nw);
+            ck_a = dalloc<double>(RN * (size_t)nw);
+            ck_j = dalloc<int32_t>(RN * (size_t)nw);
+            ck_cap = nw;
+        }
+        ck_uf.assign((size_t)(nw * R), 0.0);
+        const int64_t refills0 = refills;
+        for (int64_t w = 0; w < nw; ++w) {
+            const size_t o = (size_t)w * RN;
+            HIPC(hipMemcpyAsync(ck_u + o, u, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(ck_v + o, v, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(ck_a + o, a, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(ck_j + o, jloc, sizeof(int32_t) * RN, hipMemcpyDeviceToDevice, stream));
+            for (int64_t r = 0; r < R; ++r) {
+                ck_uf[(size_t)(w * R + r)] = hrs[(size_t)r].u_frame;
+            }
+            const uint64_t nsteps = std::min<uint64_t>(K, n - (uint64_t)w * K);
+            steps(nsteps, flow, v_frame, nsteps, false, false);
+        }
+        if (refills != refills0) {
+            throw Error("the trajectory left the landscape window (a refill was needed); the adjoint needs "
+                        "a larger nchunk");
+        }
+        adj_n = (int64_t)n;
+        adj_K = (int64_t)K;
+        adj_nw = nw;
+        adj_flow = flow ? 1 : 0;
+        adj_vf = v_frame;
+        adj_valid = true;
+    }
+
+    // cotangents of (u, v, a) after the last adjoint_forward -> cotangents of
+    // (u, v, a) before it, d/d(mu, k, k_frame, eta) [R][4] and d/dy
+    // [R][N][nchunk] (or null); all device pointers
+    void adjoint_backward(const double* gu, const double* gv, const double* ga, double* gu0, double* gv0,
+                          double* ga0, double* gpar, double* gy)
+    {
+        if (!adj_valid) {
+            throw Error("no adjoint forward run to differentiate");
+        }
+        adjoint_check();
+        const size_t RN = (size_t)total;
+        if (adj_K > tp_cap) {
+            dfree(tp_u);
+            dfree(tp_v);
+            dfree(tp_a);
+            dfree(tp_j);
+            dfree(tp_uf);
+            tp_u = dalloc<double>(RN * (size_t)adj_K);
+            tp_v = dalloc<double>(RN * (size_t)adj_K);
+            tp_a = dalloc<double>(RN * (size_t)adj_K);
+            tp_j = dalloc<int32_t>(RN * (size_t)adj_K);
+            tp_uf = dalloc<double>((size_t)(R * adj_K));
+            tp_cap = adj_K;
+        }
+        if (!wk_u) {
+            wk_u = dalloc<double>(RN);
+            wk_v = dalloc<double>(RN);
+            wk_a = dalloc<double>(RN);
+            wk_j = dalloc<int32_t>(RN);
+            wk_uf = dalloc<double>((size_t)R);
+            ad_u = dalloc<double>(RN);
+            ad_v = dalloc<double>(RN);
+            ad_a = dalloc<double>(RN);
+            ad_par = dalloc<double>((size_t)R * 4);
+            ad_err = dalloc<int32_t>(1);
+        }
+        // launch shape: fewest particles per thread that fits
+        int vp[8], vm[8];
+        const int nv = fq::adj_variants(vp, vm, 8);
+        int ppt = 0, threads = 0;
+        for (int i = 0; i < nv; ++i) {
+            const int64_t need = (N + vp[i] - 1) / vp[i];
+            const int thr = (int)(((need + 63) / 64) * 64);
+            if (thr <= vm[i]) {
+                ppt = vp[i];
+                threads = thr;
+                break;
+            }
+        }
+        if (!ppt) {
+            throw Error("no adjoint kernel for this system size");
+        }
+        if (gy) {
+            HIPC(hipMemsetAsync(gy, 0, sizeof(double) * RN * (size_t)nchunk, stream));
+        }
+        HIPC(hipMemsetAsync(ad_par, 0, sizeof(double) * (size_t)R * 4, stream));
+        HIPC(hipMemsetAsync(ad_err, 0, sizeof(int32_t), stream));
+        HIPC(hipMemcpyAsync(ad_u, gu, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipMemcpyAsync(ad_v, gv, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipMemcpyAsync(ad_a, ga, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        const fq::AdjBuffers ab = adj_buffers(gy);
+        for (int64_t w = adj_nw - 1; w >= 0; --w) {
+            const size_t o = (size_t)w * RN;
+            const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
+            HIPC(hipMemcpyAsync(wk_u, ck_u + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(wk_v, ck_v + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(wk_a, ck_a + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(wk_j, ck_j + o, sizeof(int32_t) * RN, hipMemcpyDeviceToDevice, stream));
+            HIPC(hipMemcpyAsync(wk_uf, ck_uf.data() + w * R, sizeof(double) * (size_t)R, hipMemcpyHostToDevice,
+                                stream));
+            if (!fq::launch_adj_record(ppt, threads, views, ab, nsteps, adj_flow, adj_vf, stream)) {
+                throw Error("adjoint record kernel not available");
+            }
+            HIPC(hipGetLastError());
+            if (!fq::launch_adj_reverse(ppt, threads, views, ab, nsteps, stream)) {
+                throw Error("adjoint reverse kernel not available");
+            }
+            HIPC(hipGetLastError());
+            // the host-side frame positions of the next window's copy must stay
+            // alive until it ran: synchronise per window
+            HIPC(hipStreamSynchronize(stream));
+        }
+        int32_t err = 0;
+        d2h(&err, ad_err, 1);
+        if (err) {
+            throw Error("a well walk left the landscape window while recomputing the trajectory");
+        }
+        HIPC(hipMemcpyAsync(gu0, ad_u, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipMemcpyAsync(gv0, ad_v, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipMemcpyAsync(ga0, ad_a, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipMemcpyAsync(gpar, ad_par, sizeof(double) * (size_t)R * 4, hipMemcpyDeviceToDevice, stream));
+        HIPC(hipStreamSynchronize(stream));
+    }
+
+    // per-replica mu, k_interactions, k_frame, eta (the parameters the adjoint
+    // differentiates); forces are recomputed
+    void set_parameters(const double* par)
+    {
+        for (int64_t r = 0; r < R; ++r) {
+            fq::SysView& S = hviews[(size_t)r];
+            S.mu = par[4 * r + 0];
+            S.k = par[4 * r + 1];
+            S.k_frame = par[4 * r + 2];
+            S.eta = par[4 * r + 3];
+        }
+        h2d(views, hviews.data(), (size_t)R);
+        refresh();
+        adj_valid = false;
+    }
+
+    void get_parameters(double* par) const
+    {
+        for (int64_t r = 0; r < R; ++r) {
+            const fq::SysView& S = hviews[(size_t)r];
+            par[4 * r + 0] = S.mu;
+            par[4 * r + 1] = S.k;
+            par[4 * r + 2] = S.k_frame;
+            par[4 * r + 3] = S.eta;
+        }
     }
 
     void set_lr_method(int method)
@@ -1708,6 +1947,93 @@ int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out)
 void* fqsb_batch_stream(fqsb_batch* b)
 {
     return (void*)b->g->stream;
+}
+
+// --- batch: parameters, device-side access, adjoint ----------------------------
+
+int fqsb_batch_set_parameters(fqsb_batch* b, const double* params)
+{
+    return guard([&] { b->g->set_parameters(params); });
+}
+
+int fqsb_batch_get_parameters(const fqsb_batch* b, double* params)
+{
+    b->g->get_parameters(params);
+    return 0;
+}
+
+uint64_t fqsb_batch_nchunk(const fqsb_batch* b)
+{
+    return (uint64_t)b->g->nchunk;
+}
+
+int fqsb_batch_get_device(fqsb_batch* b, int field, double* out)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        HIPC(hipMemcpyAsync(out, g.field(field), sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
+                            g.stream));
+        HIPC(hipStreamSynchronize(g.stream));
+    });
+}
+
+int fqsb_batch_set_device(fqsb_batch* b, int field, const double* in)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        if (field != FQSB_U && field != FQSB_V && field != FQSB_A) {
+            throw Error("field is read-only");
+        }
+        HIPC(hipMemcpyAsync(g.field(field), in, sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
+                            g.stream));
+        HIPC(hipStreamSynchronize(g.stream));
+        if (field == FQSB_U) {
+            g.updated_u();
+        }
+        else if (field == FQSB_V) {
+            g.updated_v();
+        }
+        g.adj_valid = false;
+    });
+}
+
+int fqsb_batch_chunk_data_device(fqsb_batch* b, double* out)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        if (!g.uniform) {
+            throw Error("chunk data of the batch needs replicas of one shape");
+        }
+        HIPC(hipMemcpyAsync(out, g.y, sizeof(double) * (size_t)g.total * (size_t)g.nchunk,
+                            hipMemcpyDeviceToDevice, g.stream));
+        HIPC(hipStreamSynchronize(g.stream));
+    });
+}
+
+int fqsb_batch_set_chunk_data_device(fqsb_batch* b, const double* in)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        if (!g.uniform) {
+            throw Error("chunk data of the batch needs replicas of one shape");
+        }
+        HIPC(hipMemcpyAsync(g.y, in, sizeof(double) * (size_t)g.total * (size_t)g.nchunk, hipMemcpyDeviceToDevice,
+                            g.stream));
+        HIPC(hipStreamSynchronize(g.stream));
+        g.refresh();
+        g.adj_valid = false;
+    });
+}
+
+int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame)
+{
+    return guard([&] { b->g->adjoint_forward(n, window, flow != 0, v_frame); });
+}
+
+int fqsb_batch_adjoint_backward_device(fqsb_batch* b, const double* gu, const double* gv, const double* ga,
+                                       double* gu0, double* gv0, double* ga0, double* gparams, double* gy)
+{
+    return guard([&] { b->g->adjoint_backward(gu, gv, ga, gu0, gv0, ga0, gparams, gy); });
 }
 
 // --- ragged set ----------------------------------------------------------------

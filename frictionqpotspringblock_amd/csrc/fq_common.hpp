@@ -153,6 +153,29 @@ struct LrBuffers {
     int nsplit; // number of partial products
 };
 
+// Buffers of the adjoint of a batch (fq_adjoint.hpp); [R*N] arrays are
+// replica after replica.
+struct AdjBuffers {
+    double* tu;     // tape [K][R*N]: u at the start of step n
+    double* tv;     // tape: v
+    double* ta;     // tape: a
+    int32_t* tj;    // tape: chunk-local well index used by step n
+    double* tuf;    // tape [K][R]: frame position of step n
+    double* wu;     // work state [R*N] (checkpoint in, end of window out)
+    double* wv;
+    double* wa;
+    int32_t* wj;
+    double* wuf;    // [R] frame position at the window start
+    double* au;     // adjoint state [R*N] (cotangent of u, v, a)
+    double* av;
+    double* aa;
+    double* gpar;   // [R][4] accumulated d/d(mu, k_interactions, k_frame, eta)
+    double* gy;     // [R][N][nchunk] accumulated d/dy, or null
+    int32_t* err;   // set to 1 if a well walk left the landscape window
+    int64_t N;
+    int64_t R;
+};
+
 struct RunArgs {
     int32_t mode;
     int32_t flow;          // flowSteps: u_frame += v_frame * dt before each step

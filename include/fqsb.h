@@ -244,6 +244,36 @@ int fqsb_batch_chunk_index_at_align(fqsb_batch* b, int64_t* out);
 /* The HIP stream (hipStream_t) the batch launches on. */
 void* fqsb_batch_stream(fqsb_batch* b);
 
+/* Per-replica parameters [R][4] = (mu, k_interactions, k_frame, eta), host
+ * arrays; setting them recomputes the forces. */
+int fqsb_batch_set_parameters(fqsb_batch* b, const double* params);
+int fqsb_batch_get_parameters(const fqsb_batch* b, double* params);
+uint64_t fqsb_batch_nchunk(const fqsb_batch* b);
+/* Device-side access (pointers on the batch's device, e.g. from torch):
+ * fields [R, N] (setting u or v recomputes forces as fqsb_batch_set), the
+ * yield-position chunks [R, N, nchunk] (setting them realigns and recomputes
+ * the forces). */
+int fqsb_batch_get_device(fqsb_batch* b, int field, double* out);
+int fqsb_batch_set_device(fqsb_batch* b, int field, const double* in);
+int fqsb_batch_chunk_data_device(fqsb_batch* b, double* out);
+int fqsb_batch_set_chunk_data_device(fqsb_batch* b, const double* in);
+
+/* Adjoint (no reference counterpart) of timeSteps / flowSteps (detail.h:
+ * 1539-1583, 1637-1645) for batches of Cuspy + Laplace1d replicas with
+ * N <= 4096 and no external forcing. fqsb_batch_adjoint_forward runs n steps
+ * (flow != 0: flowSteps at v_frame) from the current state, keeping a
+ * checkpoint every `window` steps; it fails if the trajectory needs a
+ * landscape refill. fqsb_batch_adjoint_backward_device maps the cotangents
+ * (gu, gv, ga) of the final u, v, a to those of the initial state (gu0, gv0,
+ * ga0), the derivatives d/d(mu, k_interactions, k_frame, eta) per replica
+ * (gparams [R][4]) and d/dy of the yield-position chunks (gy [R][N][nchunk],
+ * NULL to skip), differentiating the exact discrete step with every well
+ * index held at the value the trajectory used. All pointers are device
+ * pointers. */
+int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame);
+int fqsb_batch_adjoint_backward_device(fqsb_batch* b, const double* gu, const double* gv, const double* ga,
+                                       double* gu0, double* gv0, double* ga0, double* gparams, double* gy);
+
 /* ------------------------------------------------------------------------
  * Ragged set (no reference counterpart): n systems of any classes, sizes and
  * parameters (descs[i] as for fqsb_create), advanced together. Systems that
