@@ -8,6 +8,13 @@ increments in chunks of 200 wells, driven by flowSteps at v_frame = 1e-3. One
 bench step = one resident launch of 1000 flowSteps over all replicas (4.19e9
 particle-steps); the default --steps 100 is the full 1e5-step run.
 
+--workload adjoint: R = 64 Cuspy-Laplace replicas of N = 4096 (parameters as
+the flow workload, random initial velocities); one bench step = one forward
+pass of T = 2000 timeSteps with a checkpoint every 100 steps plus its adjoint
+(torch.autograd backward) for loss = mean over replicas of |u(T) - u_t|^2,
+where u_t comes from replicas with mu * 1.05, k * 0.95, eta * 1.1; gradients
+with respect to u0, v0, a0, (mu, k, k_frame, eta) and the yield positions.
+
 --workload longrange: R = 256 replicas of N = 8192 particles with LongRange1d
 interactions (alpha = 1.5, k = 1) and random normal forcing (mean 0, stddev
 0.1, dinc = dinc_init = 10), otherwise as above; timeSteps with the
@@ -43,6 +50,12 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 LR_R = 256
 LR_N = 8192
 LR_BLOCK = 100
+
+AD_R = 64
+AD_N = 4096
+AD_T = 2000
+AD_K = 100
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "particle-steps/sec batched vs plain on 1 GPU (FP64) + % HBM/MFMA roofline"
 
 PARAMS = dict(
@@ -232,6 +245,127 @@ def run_longrange(args):
     }
 
 
+def ad_cpu_baseline(replicas=16, target_s=10.0):
+    """The oracle's forward timeSteps of the adjoint workload on the host (one
+    replica per thread). The oracle has no adjoint; a CPU forward + adjoint
+    costs about three forward passes, so this rate is an upper bound."""
+    from oracle import pyoracle
+
+    pyoracle.lib()
+    cores = min(16, os.cpu_count() or 1)
+    replicas = max(replicas, cores)
+    rng = np.random.default_rng(0)
+    systems = []
+    for r in range(replicas):
+        s = pyoracle.Line1d.System_Cuspy_Laplace(**dict(PARAMS, seed=r * AD_N))
+        s.v = rng.normal(size=AD_N) * 0.5
+        systems.append(s)
+
+    def sweep(n):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+            list(ex.map(lambda s: s.timeSteps(n), systems))
+        return time.perf_counter() - t0
+
+    pilot = 100
+    dt_pilot = sweep(pilot)
+    steps = max(pilot, int(pilot * target_s / max(dt_pilot, 1e-3)))
+    dt = sweep(steps)
+    return {
+        "value": replicas * AD_N * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{replicas} replicas x N={AD_N} x {steps} forward timeSteps (no adjoint on the CPU), "
+        f"one replica per thread ({dt:.1f} s)",
+    }
+
+
+def run_adjoint(args):
+    import torch
+
+    from frictionqpotspringblock_amd import adjoint as A
+    from frictionqpotspringblock_amd.batch import Batch
+
+    torch.cuda.init()
+    kw = dict(PARAMS, k_frame=1.0 / AD_N, shape=[AD_N])
+    batch = Batch(AD_R, seed_stride=AD_N, potential="Cuspy", interactions="Laplace1d", **kw)
+    target = Batch(AD_R, seed_stride=AD_N, potential="Cuspy", interactions="Laplace1d", **kw)
+    rng = np.random.default_rng(0)
+    dev = "cuda"
+    u0 = A.get_field(batch, "u")
+    v0 = torch.tensor(rng.normal(size=(AD_R, AD_N)) * 0.5, device=dev)
+    a0 = torch.zeros((AD_R, AD_N), dtype=torch.float64, device=dev)
+    par = A.parameters(batch)
+    par_t = par.clone()
+    par_t[:, 0] *= 1.05
+    par_t[:, 1] *= 0.95
+    par_t[:, 3] *= 1.1
+    y = A.chunk_data(batch)
+    with torch.no_grad():
+        u_t, _, _ = A.run(target, AD_T, AD_K, u0, v0, a0, par_t, y)
+    del target
+
+    def step():
+        leaves = [x.clone().requires_grad_(True) for x in (u0, v0, a0, par, y)]
+        uT, _, _ = A.run(batch, AD_T, AD_K, *leaves)
+        loss = ((uT - u_t) ** 2).sum(dim=1).mean()
+        loss.backward()
+        return loss.item(), leaves
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, leaves = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rec_ms, rev_ms = A.kernel_ms(batch)
+    g = leaves[3].grad
+    assert torch.all(torch.isfinite(g))
+    value = AD_R * AD_N * AD_T * args.steps / wall
+    tape_bytes = 28.0 * AD_R * AD_N * AD_K  # u, v, a (FP64) and well index (int32) per particle-step
+    achieved = tape_bytes / (rev_ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else ad_cpu_baseline()
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Weibull yield landscape from pcg32 seeds r*{AD_N}, normal initial velocities)",
+        "config": {
+            "workload": f"adjoint: {AD_R} Line1d Cuspy-Laplace replicas x N={AD_N}, T={AD_T} timeSteps "
+            f"forward + adjoint, checkpoint every {AD_K} steps",
+            "replicas": AD_R,
+            "N": AD_N,
+            "T": AD_T,
+            "window": AD_K,
+            "loss": loss,
+            "grad_params_replica0": [float(x) for x in g[0]],
+            "record_ms_per_window": rec_ms,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_adj_reverse",
+            "kernel_ms": rev_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
 def run_flow(args):
     import torch
 
@@ -303,11 +437,11 @@ def run_flow(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["flow", "longrange"], default="flow")
+    ap.add_argument("--workload", choices=["flow", "longrange", "adjoint"], default="flow")
     ap.add_argument(
         "--steps", type=int, default=None,
         help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
-        "longrange: blocks of 100 timeSteps, default 200)",
+        "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5)",
     )
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -322,8 +456,9 @@ def main():
         )
         sys.exit(2)
     if args.steps is None:
-        args.steps = 100 if args.workload == "flow" else 200
-    line = run_flow(args) if args.workload == "flow" else run_longrange(args)
+        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5}[args.workload]
+    run = {"flow": run_flow, "longrange": run_longrange, "adjoint": run_adjoint}[args.workload]
+    line = run(args)
     print(json.dumps(line), flush=True)
 
 

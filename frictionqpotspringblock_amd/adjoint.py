@@ -118,6 +118,14 @@ class _VerletAdjoint(torch.autograd.Function):
         )
 
 
+def kernel_ms(batch):
+    """(record, reverse): average duration in ms of one window's recompute and
+    reverse kernel in the last backward pass."""
+    a, b = C.c_float(), C.c_float()
+    _check(capi().fqsb_batch_adjoint_kernel_ms(batch._h, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
 def run(batch, n, window, u0, v0, a0, params, y=None, flow=False, v_frame=0.0):
     """n steps of the batch from (u0, v0, a0) with parameters params [R, 4]
     (and yield-position chunks y [R, N, nchunk] if given); differentiable.

@@ -172,6 +172,7 @@ SIGNATURES = {
     "fqsb_batch_set_chunk_data_device": (C.c_int, [_P, C.c_void_p]),
     "fqsb_batch_adjoint_forward": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int, C.c_double]),
     "fqsb_batch_adjoint_backward_device": (C.c_int, [_P] + [C.c_void_p] * 8),
+    "fqsb_batch_adjoint_kernel_ms": (C.c_int, [_P, _fp, _fp]),
     "fqsb_ragged_create": (C.c_int, [C.POINTER(Desc), C.c_uint64, C.POINTER(_P)]),
     "fqsb_ragged_destroy": (C.c_int, [_P]),
     "fqsb_ragged_systems": (C.c_uint64, [_P]),

@@ -48,13 +48,11 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     const int32_t jmin = (int32_t)L.buffer;
     const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
 
-    // the yield positions of the current wells are re-read from the chunk
-    // (L1/L2) where needed, which keeps the registers to u, v, a, well index
-    double u[PPT], v[PPT], a[PPT];
+    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT];
     int32_t jl[PPT];
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
-        u[i] = v[i] = a[i] = 0.0;
+        u[i] = v[i] = a[i] = yl[i] = yr[i] = 0.0;
         jl[i] = 0;
         if (i < cnt) {
             const int64_t p = base + i;
@@ -62,6 +60,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
             v[i] = ab.wv[off + p];
             a[i] = ab.wa[off + p];
             jl[i] = ab.wj[off + p];
+            yl[i] = Y[p * nch + jl[i]];
+            yr[i] = Y[p * nch + jl[i] + 1];
         }
     }
     const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, inv_m = S.inv_m, kf = S.k_frame;
@@ -87,10 +87,10 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                 ab.ta[tn + p] = a[i];
                 u[i] = u[i] + dt * v[i] + c2 * a[i];
                 const double x = u[i];
-                const double* row = Y + p * nch;
-                int32_t j = jl[i];
-                double lo = row[j], hi = row[j + 1];
-                if (!(lo <= x && x < hi)) {
+                if (!(yl[i] <= x && x < yr[i])) {
+                    const double* row = Y + p * nch;
+                    int32_t j = jl[i];
+                    double lo = yl[i], hi = yr[i];
                     while (x >= hi && j < jmax) {
                         ++j;
                         lo = hi;
@@ -103,6 +103,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                     }
                     bad |= !(lo <= x && x < hi);
                     jl[i] = j;
+                    yl[i] = lo;
+                    yr[i] = hi;
                 }
                 ab.tj[tn + p] = jl[i];
             }
@@ -119,9 +121,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
             if (i < cnt) {
                 const double um = i == 0 ? left : u[i > 0 ? i - 1 : 0];
                 const double up = i == cnt - 1 ? right : u[i + 1 < PPT ? i + 1 : i];
-                const double* row = Y + (base + i) * nch;
                 const double fi = chain_force<INT_LAPLACE1D>(S, um, u[i], up);
-                const double fp = potential_force<POT_CUSPY>(S, u[i], row[jl[i]], row[jl[i] + 1]);
+                const double fp = potential_force<POT_CUSPY>(S, u[i], yl[i], yr[i]);
                 const double ff = kf * (uf - u[i]);
                 const double sum3 = ff + fp + fi;
                 const double vn = v[i], an = a[i];
@@ -187,11 +188,13 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
     const double hmu = 0.5 * mu;
 
     // U, V, A: cotangents of u, v, a after the step being reversed
-    double U[PPT], V[PPT], A[PPT], acc[PPT];
+    // jc, ymc: the well of the current tape step and its mid-point (read from
+    // the chunk only when the well changes)
+    double U[PPT], V[PPT], A[PPT], acc[PPT], ymc[PPT];
     int32_t jc[PPT];
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
-        U[i] = V[i] = A[i] = acc[i] = 0.0;
+        U[i] = V[i] = A[i] = acc[i] = ymc[i] = 0.0;
         jc[i] = -1;
         if (i < cnt) {
             const int64_t p = base + i;
@@ -243,8 +246,9 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
                 if (j != jc[i]) {
                     flush(i);
                     jc[i] = j;
+                    ymc[i] = 0.5 * (Y[p * nch + j] + Y[p * nch + j + 1]);
                 }
-                const double ym = 0.5 * (Y[p * nch + j] + Y[p * nch + j + 1]);
+                const double ym = ymc[i];
                 const double v = ab.tv[tn + p], a = ab.ta[tn + p];
                 // forward quantities of the step
                 const double g = kf * (uf - u1) + (ym - u1) * mu + lu * k;

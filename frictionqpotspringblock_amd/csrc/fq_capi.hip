@@ -109,6 +109,8 @@ struct Group {
     double *ad_u = nullptr, *ad_v = nullptr, *ad_a = nullptr, *ad_par = nullptr;
     int32_t* ad_err = nullptr;
     int64_t ck_cap = 0, tp_cap = 0; // windows / steps the buffers hold
+    hipEvent_t eva[3] = {nullptr, nullptr, nullptr}; // around record and reverse launches
+    float adj_record_ms = 0.0f, adj_reverse_ms = 0.0f; // per launch, last backward
     int pot = 0, inter = 0, ext = 0, minim = 0;
     double *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *f_pot = nullptr, *f_int = nullptr,
            *f_frame = nullptr, *f_damp = nullptr, *f_th = nullptr, *pref = nullptr, *y = nullptr;
@@ -422,6 +424,12 @@ struct Group {
         }
         ck_cap = tp_cap = 0;
         adj_valid = false;
+        for (auto& e : eva) {
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        }
     }
 
     fq::AdjBuffers adj_buffers(double* gy) const
@@ -568,6 +576,12 @@ struct Group {
         HIPC(hipMemcpyAsync(ad_v, gv, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
         HIPC(hipMemcpyAsync(ad_a, ga, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
         const fq::AdjBuffers ab = adj_buffers(gy);
+        for (auto& e : eva) {
+            if (!e) {
+                HIPC(hipEventCreate(&e));
+            }
+        }
+        double rec_ms = 0.0, rev_ms = 0.0;
         for (int64_t w = adj_nw - 1; w >= 0; --w) {
             const size_t o = (size_t)w * RN;
             const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
@@ -577,18 +591,28 @@ struct Group {
             HIPC(hipMemcpyAsync(wk_j, ck_j + o, sizeof(int32_t) * RN, hipMemcpyDeviceToDevice, stream));
             HIPC(hipMemcpyAsync(wk_uf, ck_uf.data() + w * R, sizeof(double) * (size_t)R, hipMemcpyHostToDevice,
                                 stream));
+            HIPC(hipEventRecord(eva[0], stream));
             if (!fq::launch_adj_record(ppt, threads, views, ab, nsteps, adj_flow, adj_vf, stream)) {
                 throw Error("adjoint record kernel not available");
             }
             HIPC(hipGetLastError());
+            HIPC(hipEventRecord(eva[1], stream));
             if (!fq::launch_adj_reverse(ppt, threads, views, ab, nsteps, stream)) {
                 throw Error("adjoint reverse kernel not available");
             }
             HIPC(hipGetLastError());
+            HIPC(hipEventRecord(eva[2], stream));
             // the host-side frame positions of the next window's copy must stay
             // alive until it ran: synchronise per window
             HIPC(hipStreamSynchronize(stream));
+            float e0 = 0.0f, e1 = 0.0f;
+            HIPC(hipEventElapsedTime(&e0, eva[0], eva[1]));
+            HIPC(hipEventElapsedTime(&e1, eva[1], eva[2]));
+            rec_ms += e0;
+            rev_ms += e1;
         }
+        adj_record_ms = (float)(rec_ms / (double)adj_nw);
+        adj_reverse_ms = (float)(rev_ms / (double)adj_nw);
         int32_t err = 0;
         d2h(&err, ad_err, 1);
         if (err) {
@@ -2023,6 +2047,13 @@ int fqsb_batch_set_chunk_data_device(fqsb_batch* b, const double* in)
         g.refresh();
         g.adj_valid = false;
     });
+}
+
+int fqsb_batch_adjoint_kernel_ms(const fqsb_batch* b, float* record_ms, float* reverse_ms)
+{
+    *record_ms = b->g->adj_record_ms;
+    *reverse_ms = b->g->adj_reverse_ms;
+    return 0;
 }
 
 int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame)

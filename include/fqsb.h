@@ -273,6 +273,9 @@ int fqsb_batch_set_chunk_data_device(fqsb_batch* b, const double* in);
 int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame);
 int fqsb_batch_adjoint_backward_device(fqsb_batch* b, const double* gu, const double* gv, const double* ga,
                                        double* gu0, double* gv0, double* ga0, double* gparams, double* gy);
+/* Average duration (ms) of one window's recompute (record) and reverse
+ * kernel in the last backward pass (HIP events on the batch's stream). */
+int fqsb_batch_adjoint_kernel_ms(const fqsb_batch* b, float* record_ms, float* reverse_ms);
 
 /* ------------------------------------------------------------------------
  * Ragged set (no reference counterpart): n systems of any classes, sizes and
