@@ -47,7 +47,9 @@ HIP_SOURCES = [
     "fq_stream.hip",
     "fq_longrange.hip",
     "fq_adjoint.hip",
-    "fq_res_cuspy.hip",
+    "fq_res_cuspy_laplace.hip",
+    "fq_res_cuspy_chain.hip",
+    "fq_res_cuspy_field.hip",
     "fq_res_semismooth.hip",
     "fq_res_smooth.hip",
 ]

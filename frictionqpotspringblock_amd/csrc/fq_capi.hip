@@ -58,12 +58,16 @@ int guard(F&& f)
     }
 }
 
+// Zeroed device memory. The fill runs on the null stream, which the groups'
+// non-blocking streams are not ordered after, so it is completed here before
+// any kernel can touch the memory.
 template <class T>
 T* dalloc(size_t n)
 {
     void* p = nullptr;
     HIPC(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
-    HIPC(hipMemset(p, 0, sizeof(T) * (n ? n : 1)));
+    HIPC(hipMemsetAsync(p, 0, sizeof(T) * (n ? n : 1), nullptr));
+    HIPC(hipStreamSynchronize(nullptr));
     return static_cast<T*>(p);
 }
 
@@ -125,6 +129,9 @@ struct Group {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int ppt = 0, maxt = 0, threads = 0;
     bool streaming = false; // systems too large for a resident workgroup
+    // FQSB_GENERAL_KERNEL=1: timeSteps / flowSteps on k_resident instead of
+    // k_flow (the two give the same results; a switch for comparisons)
+    bool general_kernel = false;
     double *u2 = nullptr, *v2 = nullptr, *a2 = nullptr, *yl = nullptr, *yr = nullptr, *yl2 = nullptr, *yr2 = nullptr;
     int32_t* jl2 = nullptr;
     size_t lds_limit = 65536;
@@ -330,6 +337,14 @@ struct Group {
             return;
         }
         size_t lds = fq::resident_lds(inter, N, threads);
+        // timeSteps / flowSteps of chains: the pipelined k_flow (same results)
+        const bool chain = inter == FQSB_INT_NONE || inter == FQSB_INT_LAPLACE1D || inter == FQSB_INT_QUARTIC1D ||
+                           inter == FQSB_INT_QUARTICGRADIENT1D;
+        if (ra.mode == fq::MODE_STEPS && chain && !general_kernel &&
+            fq::launch_flow(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream)) {
+            HIPC(hipGetLastError());
+            return;
+        }
         if (!fq::launch_resident(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream)) {
             throw Error("no resident kernel variant for this launch shape");
         }
@@ -993,6 +1008,9 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds)
     hipDeviceProp_t prop;
     HIPC(hipGetDeviceProperties(&prop, dev));
     g->lds_limit = prop.sharedMemPerBlock - 2048;
+    if (const char* gk = std::getenv("FQSB_GENERAL_KERNEL")) {
+        g->general_kernel = gk[0] == '1';
+    }
     HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIPC(hipEventCreate(&g->ev0));
     HIPC(hipEventCreate(&g->ev1));

@@ -110,10 +110,12 @@ __device__ __forceinline__ int32_t chunk_align_slow_inl(const Landscape& L, int6
     if (j > n - 2) {
         j = n - 2;
     }
-    while (u >= y[j + 1]) {
+    // bounded walk: a row that is not increasing (which only corrupt data can
+    // produce) never leads outside the window
+    while (j < n - 2 && u >= y[j + 1]) {
         ++j;
     }
-    while (u < y[j]) {
+    while (j > 0 && u < y[j]) {
         --j;
     }
     if (j < b || j > n - 2 - b) {
