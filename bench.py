@@ -447,7 +447,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument(
+        "--copy-probe", action="store_true",
+        help="only run a plain device copy of 1 GiB three times (calibration of HBM byte counters) and exit",
+    )
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": x.numel() * 8, "copy_probe_bytes_written": y.numel() * 8}))
+        return
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: replicas are independent and would shard across GPUs "

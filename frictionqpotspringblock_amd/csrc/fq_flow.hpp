@@ -63,11 +63,17 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     double* h_first = s_dyn;        // [2][T] first slip of each thread
     double* h_last = s_dyn + 2 * T; // [2][T] last slip of each thread
 
+    // Registers i >= cnt are padding: zero state (reset after every step) in an
+    // unbounded well, so they never walk, never request a refill, never set
+    // the NaN flag and never publish; the hot loops run over all PPT registers
+    // without per-particle branches and only the stores are guarded.
     double u[PPT], un[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], ym[PPT], fth[PPT];
     int32_t jl[PPT];
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
-        u[i] = un[i] = v[i] = a[i] = yl[i] = yr[i] = ym[i] = fth[i] = 0.0;
+        u[i] = un[i] = v[i] = a[i] = ym[i] = fth[i] = 0.0;
+        yl[i] = -__builtin_inf();
+        yr[i] = __builtin_inf();
         jl[i] = 0;
         if (i < cnt) {
             const int64_t p = base + i;
@@ -136,21 +142,26 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     // exchange buffer it&1
     auto prepare = [&](int it) {
         const int b = it & 1;
-        int refill = 0;
+        int out = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                un[i] = u[i] + dt * v[i] + c2 * a[i];
+            un[i] = u[i] + dt * v[i] + c2 * a[i];
+            out |= !(yl[i] <= un[i] && un[i] < yr[i]);
+        }
+        if (out) { // rare: some slip leaves its well
+            int refill = 0;
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
                 const double x = un[i];
                 if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
                     refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
                 }
             }
+            if (refill) {
+                s_refill[b] = it;
+            }
         }
         publish(un, b);
-        if (refill) {
-            s_refill[b] = it;
-        }
     };
 
     // the current slips into buffer 0 (forces of the final state when no step
@@ -181,7 +192,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     auto passes = [&](auto unit, int b) {
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
+            {
                 const double fi = interaction(i, b);
                 const double fp = potential_force_mid<POT>(S, u[i], yl[i], yr[i], ym[i]);
                 const double ff = kf * (uf - u[i]);
@@ -265,13 +276,17 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
 
         // commit the slips and walk to their wells (reachable: checked when
         // they were prepared; a non-finite slip keeps its well)
-        int nanf = 0;
+        int nanf = 0, out = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                u[i] = un[i];
+            u[i] = un[i];
+            nanf |= __builtin_isnan(u[i]);
+            out |= !(yl[i] <= u[i] && u[i] < yr[i]);
+        }
+        if (out) { // rare
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
                 const double x = u[i];
-                nanf |= __builtin_isnan(x);
                 if (!(yl[i] <= x && x < yr[i])) {
                     const double* row = Y + (base + i) * nch;
                     int32_t j = jl[i];
@@ -304,6 +319,15 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
         }
         else {
             passes(std::false_type{}, b);
+        }
+        if (cnt < PPT) { // padding registers stay at rest (their forces are not meaningful)
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i >= cnt) {
+                    v[i] = 0.0;
+                    a[i] = 0.0;
+                }
+            }
         }
         prepare(it + 1);
     }
