@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
         return;
     }
     const int64_t room = rs->target - rs->steps;
-    const int limit = (int)(room < ra.max_steps ? (room > 0 ? room : 0) : ra.max_steps);
+    const int limit = uniform_i((int)(room < ra.max_steps ? (room > 0 ? room : 0) : ra.max_steps));
 
     __shared__ int s_nan[2];
     __shared__ int s_refill[2];
@@ -140,6 +140,9 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
 
     // slips of step `it` into un, their reachability, and publication into
     // exchange buffer it&1
+    // pend: some slip of this thread leaves its well (or is NaN): the commit
+    // of the step walks and checks only then
+    int pend = 0;
     auto prepare = [&](int it) {
         const int b = it & 1;
         int out = 0;
@@ -148,6 +151,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             un[i] = u[i] + dt * v[i] + c2 * a[i];
             out |= !(yl[i] <= un[i] && un[i] < yr[i]);
         }
+        pend = out;
         if (out) { // rare: some slip leaves its well
             int refill = 0;
 #pragma unroll
@@ -276,17 +280,16 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
 
         // commit the slips and walk to their wells (reachable: checked when
         // they were prepared; a non-finite slip keeps its well)
-        int nanf = 0, out = 0;
+        int nanf = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             u[i] = un[i];
-            nanf |= __builtin_isnan(u[i]);
-            out |= !(yl[i] <= u[i] && u[i] < yr[i]);
         }
-        if (out) { // rare
+        if (pend) { // rare
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const double x = u[i];
+                nanf |= __builtin_isnan(x);
                 if (!(yl[i] <= x && x < yr[i])) {
                     const double* row = Y + (base + i) * nch;
                     int32_t j = jl[i];
