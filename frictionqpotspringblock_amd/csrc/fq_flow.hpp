@@ -32,7 +32,7 @@ FQ_HD double potential_force_mid(const SysView& S, double u, double yl, double y
     }
 }
 
-template <int POT, int INT, int EXT, int PPT, int MAXT>
+template <int POT, int INT, int EXT, int PPT, int MAXT, bool UNIT>
 __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views, RunArgs ra)
 {
     static_assert(is_chain<INT>(), "k_flow handles chains");
@@ -63,21 +63,26 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     double* h_first = s_dyn;        // [2][T] first slip of each thread
     double* h_last = s_dyn + 2 * T; // [2][T] last slip of each thread
 
-    // Registers i >= cnt are padding: zero state (reset after every step) in an
-    // unbounded well, so they never walk, never request a refill, never set
-    // the NaN flag and never publish; the hot loops run over all PPT registers
-    // without per-particle branches and only the stores are guarded.
-    double u[PPT], un[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], ym[PPT], fth[PPT];
+    // Slips live in two register arrays used in turn: one holds the state of
+    // the current step, the other the slips prepared for the next step (and,
+    // before they are committed, the state of the previous one). The step
+    // loop is unrolled by two so that the arrays swap roles without copies.
+    // Registers i >= cnt are padding: zero state (reset after every step) in
+    // an unbounded well, so they never walk, never request a refill, never
+    // set the NaN flag and never publish; the hot loops run over all PPT
+    // registers without per-particle branches and only the stores are
+    // guarded.
+    double sa[PPT], sb[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], ym[PPT], fth[PPT];
     int32_t jl[PPT];
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
-        u[i] = un[i] = v[i] = a[i] = ym[i] = fth[i] = 0.0;
+        sa[i] = sb[i] = v[i] = a[i] = ym[i] = fth[i] = 0.0;
         yl[i] = -__builtin_inf();
         yr[i] = __builtin_inf();
         jl[i] = 0;
         if (i < cnt) {
             const int64_t p = base + i;
-            u[i] = S.u[p];
+            sa[i] = S.u[p];
             v[i] = S.v[p];
             a[i] = S.a[p];
             jl[i] = L.jloc[p];
@@ -95,7 +100,6 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     const double hdt = 0.5 * dt;
     const double meta = -S.eta;
     const double inv_m = S.inv_m;
-    const bool unit_mass = inv_m == 1.0;
     const double kf = S.k_frame;
     const double dv = ra.v_frame * dt;
 
@@ -138,44 +142,39 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
         }
     };
 
-    // slips of step `it` into un, their reachability, and publication into
-    // exchange buffer it&1
-    // pend: some slip of this thread leaves its well (or is NaN): the commit
-    // of the step walks and checks only then
+    // slips of step `it` from the state x into y, their reachability, and
+    // publication into exchange buffer it&1; pend: some slip of this thread
+    // leaves its well (or is NaN), so that the commit of the step walks and
+    // checks only then
     int pend = 0;
-    auto prepare = [&](int it) {
+    auto prepare = [&](int it, const double(&x)[PPT], double(&y)[PPT]) {
         const int b = it & 1;
         int out = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            un[i] = u[i] + dt * v[i] + c2 * a[i];
-            out |= !(yl[i] <= un[i] && un[i] < yr[i]);
+            y[i] = x[i] + dt * v[i] + c2 * a[i];
+            out |= !(yl[i] <= y[i] && y[i] < yr[i]);
         }
         pend = out;
         if (out) { // rare: some slip leaves its well
             int refill = 0;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                const double x = un[i];
-                if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
-                    refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
+                const double z = y[i];
+                if (!(yl[i] <= z && z < yr[i]) && __builtin_isfinite(z)) {
+                    refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], z, jmin, jmax);
                 }
             }
             if (refill) {
                 s_refill[b] = it;
             }
         }
-        publish(un, b);
+        publish(y, b);
     };
-
-    // the current slips into buffer 0 (forces of the final state when no step
-    // is taken), the slips of step 1 into buffer 1
-    publish(u, 0);
-    prepare(1);
 
     auto left_of = [&](int b) -> double { return h_last[b * T + (t == 0 ? Tact - 1 : t - 1)]; };
     auto right_of = [&](int b) -> double { return h_first[b * T + (t == Tact - 1 ? 0 : t + 1)]; };
-    auto interaction = [&](int i, int b) -> double {
+    auto interaction = [&](int i, int b, const double(&x)[PPT]) -> double {
         if constexpr (INT == INT_NONE) {
             return 0.0;
         }
@@ -184,58 +183,34 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 if (q == i) {
-                    um = (q == 0) ? left_of(b) : u[q > 0 ? q - 1 : 0];
-                    up = (q == cnt - 1) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
+                    um = (q == 0) ? left_of(b) : x[q > 0 ? q - 1 : 0];
+                    up = (q == cnt - 1) ? right_of(b) : x[q + 1 < PPT ? q + 1 : q];
                 }
             }
-            return chain_force<INT>(S, um, u[i], up);
-        }
-    };
-
-    // the reference's three velocity / acceleration passes (detail.h:1552-1565)
-    auto passes = [&](auto unit, int b) {
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            {
-                const double fi = interaction(i, b);
-                const double fp = potential_force_mid<POT>(S, u[i], yl[i], yr[i], ym[i]);
-                const double ff = kf * (uf - u[i]);
-                const double sum3 = ff + fp + fi;
-                const double vn = v[i], an = a[i];
-                double vv = vn + dt * an;
-                double f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                double aa = decltype(unit)::value ? f : f * inv_m;
-                vv = vn + hdt * (an + aa);
-                f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                aa = decltype(unit)::value ? f : f * inv_m;
-                vv = vn + hdt * (an + aa);
-                f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                aa = decltype(unit)::value ? f : f * inv_m;
-                v[i] = vv;
-                a[i] = aa;
-            }
+            return chain_force<INT>(S, um, x[i], up);
         }
     };
 
     int32_t status = ST_RUNNING;
-    int it;
-    for (it = 1;; ++it) {
+
+    // step `it` with the prepared slips in x (the previous state in y, which
+    // receives the slips of step it+1); false if the run stops before it
+    auto step = [&](int it, double(&x)[PPT], double(&y)[PPT]) -> bool {
         const int b = it & 1;
         __syncthreads(); // slips of step `it` published, flags of step it-1 set
 
         if (it > 1 && uniform_i(s_nan[(it - 1) & 1]) == it - 1) {
             status = ST_NAN;
-            break;
+            return false;
         }
         if (it > limit) {
-            break;
+            return false;
         }
         if (uniform_i(s_refill[b]) == it) {
             status = ST_REFILL;
-            break;
+            return false;
         }
 
-        // step `it`
         inc += 1;
         if (ra.flow) {
             uf = uf + dv;
@@ -278,33 +253,30 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             }
         }
 
-        // commit the slips and walk to their wells (reachable: checked when
-        // they were prepared; a non-finite slip keeps its well)
-        int nanf = 0;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            u[i] = un[i];
-        }
+        // commit: the slips in x are the state now; walk to their wells
+        // (reachable: checked when they were prepared; a non-finite slip
+        // keeps its well)
         if (pend) { // rare
+            int nanf = 0;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                const double x = u[i];
-                nanf |= __builtin_isnan(x);
-                if (!(yl[i] <= x && x < yr[i])) {
+                const double z = x[i];
+                nanf |= __builtin_isnan(z);
+                if (!(yl[i] <= z && z < yr[i])) {
                     const double* row = Y + (base + i) * nch;
                     int32_t j = jl[i];
                     double lo = yl[i], hi = yr[i];
-                    while (x >= hi && j < jmax) {
+                    while (z >= hi && j < jmax) {
                         ++j;
                         lo = hi;
                         hi = row[j + 1];
                     }
-                    while (x < lo && j > jmin) {
+                    while (z < lo && j > jmin) {
                         --j;
                         hi = lo;
                         lo = row[j];
                     }
-                    if (lo <= x && x < hi) {
+                    if (lo <= z && z < hi) {
                         jl[i] = j;
                         yl[i] = lo;
                         yr[i] = hi;
@@ -312,18 +284,32 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
                     }
                 }
             }
-        }
-        if (nanf) {
-            s_nan[b] = it;
+            if (nanf) {
+                s_nan[b] = it;
+            }
         }
 
-        if (unit_mass) {
-            passes(std::true_type{}, b);
+        // the reference's three velocity / acceleration passes (detail.h:1552-1565)
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const double fi = interaction(i, b, x);
+            const double fp = potential_force_mid<POT>(S, x[i], yl[i], yr[i], ym[i]);
+            const double ff = kf * (uf - x[i]);
+            const double sum3 = ff + fp + fi;
+            const double vn = v[i], an = a[i];
+            double vv = vn + dt * an;
+            double f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+            double aa = UNIT ? f : f * inv_m;
+            vv = vn + hdt * (an + aa);
+            f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+            aa = UNIT ? f : f * inv_m;
+            vv = vn + hdt * (an + aa);
+            f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+            aa = UNIT ? f : f * inv_m;
+            v[i] = vv;
+            a[i] = aa;
         }
-        else {
-            passes(std::false_type{}, b);
-        }
-        if (cnt < PPT) { // padding registers stay at rest (their forces are not meaningful)
+        if (cnt < PPT) { // padding registers stay at rest
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 if (i >= cnt) {
@@ -332,47 +318,65 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
                 }
             }
         }
-        prepare(it + 1);
-    }
+        prepare(it + 1, x, y);
+        return true;
+    };
 
-    const int done = it - 1; // completed steps
-    if (status == ST_RUNNING) {
-        status = ST_BUDGET;
-    }
-    // state and forces of the final configuration; its slips are in exchange
-    // buffer done&1 (published before step `done`, or initially)
-    const int fb = done & 1;
+    // state and forces of the final configuration (slips in x, `done` steps
+    // taken); its slips are in exchange buffer done&1 (published before step
+    // `done`, or initially)
+    auto finish = [&](const double(&x)[PPT], int done) {
+        if (status == ST_RUNNING) {
+            status = ST_BUDGET;
+        }
+        const int fb = done & 1;
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        if (i < cnt) {
-            const int64_t p = base + i;
-            S.u[p] = u[i];
-            S.v[p] = v[i];
-            S.a[p] = a[i];
-            L.jloc[p] = jl[i];
-            if (status != ST_NAN) {
-                const double fi = interaction(i, fb);
-                const double fp = potential_force_mid<POT>(S, u[i], yl[i], yr[i], ym[i]);
-                const double ff = kf * (uf - u[i]);
-                const double fd = meta * v[i];
-                S.f_int[p] = fi;
-                S.f_pot[p] = fp;
-                S.f_frame[p] = ff;
-                S.f_damp[p] = fd;
-                S.f[p] = EXT ? ff + fp + fi + fd + fth[i] : ff + fp + fi + fd;
-            }
-            if constexpr (EXT) {
-                S.f_th[p] = fth[i];
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t p = base + i;
+                S.u[p] = x[i];
+                S.v[p] = v[i];
+                S.a[p] = a[i];
+                L.jloc[p] = jl[i];
+                if (status != ST_NAN) {
+                    const double fi = interaction(i, fb, x);
+                    const double fp = potential_force_mid<POT>(S, x[i], yl[i], yr[i], ym[i]);
+                    const double ff = kf * (uf - x[i]);
+                    const double fd = meta * v[i];
+                    S.f_int[p] = fi;
+                    S.f_pot[p] = fp;
+                    S.f_frame[p] = ff;
+                    S.f_damp[p] = fd;
+                    S.f[p] = EXT ? ff + fp + fi + fd + fth[i] : ff + fp + fi + fd;
+                }
+                if constexpr (EXT) {
+                    S.f_th[p] = fth[i];
+                }
             }
         }
-    }
-    if (t == 0) {
-        rs->inc = inc;
-        rs->u_frame = uf;
-        rs->steps += done;
-        rs->status = status;
-        rs->rng_state = rng;
-        rs->min_next = min_next;
+        if (t == 0) {
+            rs->inc = inc;
+            rs->u_frame = uf;
+            rs->steps += done;
+            rs->status = status;
+            rs->rng_state = rng;
+            rs->min_next = min_next;
+        }
+    };
+
+    // the current slips into buffer 0 (forces of the final state when no step
+    // is taken), the slips of step 1 into buffer 1
+    publish(sa, 0);
+    prepare(1, sa, sb);
+    for (int it = 1;; it += 2) {
+        if (!step(it, sb, sa)) {
+            finish(sa, it - 1);
+            return;
+        }
+        if (!step(it + 1, sa, sb)) {
+            finish(sb, it);
+            return;
+        }
     }
 }
 
