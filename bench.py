@@ -46,6 +46,12 @@ BLOCK = 1000
 V_FRAME = 1e-3
 FLOP_PER_PARTICLE_STEP = 40.0  # Cuspy + Laplace1d Velocity-Verlet step
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+# HBM bytes per 1000-step launch of k_flow<Cuspy, Laplace1d, 4 x 1024> on this
+# workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+# each scaled by the ratio of the true bytes of a 1 GiB device copy to what
+# the same counter reads for it (reads x 2.000, writes x 1.000);
+# profiles/k_flow_cuspy_laplace_flow_R1024_N4096.md
+FLOW_TRAFFIC_BYTES = (501_500 * 2.000 + 792_000 * 1.000) * 1024
 
 LR_R = 256
 LR_N = 8192
@@ -426,9 +432,10 @@ def run_flow(args):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "k_resident<Cuspy, Laplace1d>",
+            "traffic": FLOW_TRAFFIC_BYTES if (ppt, threads) == (4, 1024) else None,
+            "kernel": f"k_flow<Cuspy, Laplace1d, {ppt} x {threads}>",
             "kernel_ms": kernel_ms,
+            "hbm_gbs": FLOW_TRAFFIC_BYTES / (kernel_ms * 1e-3) / 1e9 if (ppt, threads) == (4, 1024) else None,
         },
         "cpu_baseline": cpu,
     }
