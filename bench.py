@@ -15,6 +15,15 @@ pass of T = 2000 timeSteps with a checkpoint every 100 steps plus its adjoint
 where u_t comes from replicas with mu * 1.05, k * 0.95, eta * 1.1; gradients
 with respect to u0, v0, a0, (mu, k, k_frame, eta) and the yield positions.
 
+--workload ragged: 512 Line1d systems with N = floor(2^U(9, 16)) (seeded draw),
+potentials cycling Cuspy / SemiSmooth (kappa 1) / Smooth, 30% Quartic (a1 1,
+a2 5) and 70% Laplace (k 1) interactions, plus 8 Line2d Cuspy-Laplace 256 x
+256; one bench step = one quasistatic event of the examples' protocol on all
+systems (elastic step except for Smooth, kick, minimise), then the same number
+of events with the no-passing minimiser on the Cuspy-Laplace chains (up to
+8192 particles); value = sum over systems of N x (time steps or sweeps) per
+second.
+
 --workload longrange: R = 256 replicas of N = 8192 particles with LongRange1d
 interactions (alpha = 1.5, k = 1) and random normal forcing (mean 0, stddev
 0.1, dinc = dinc_init = 10), otherwise as above; timeSteps with the
@@ -56,6 +65,8 @@ FLOW_TRAFFIC_BYTES = (501_500 * 2.000 + 792_000 * 1.000) * 1024
 LR_R = 256
 LR_N = 8192
 LR_BLOCK = 100
+
+RG_EPS = 1e-3
 
 AD_R = 64
 AD_N = 4096
@@ -372,6 +383,150 @@ def run_adjoint(args):
     }
 
 
+def ragged_systems(n1d=512, n2d=8, side=256, nchunk=200):
+    """The ragged workload: (class name, constructor arguments) per system."""
+    rng = np.random.default_rng(0)
+    Ns = np.floor(2.0 ** rng.uniform(9, 16, n1d)).astype(int)
+    quartic = rng.uniform(size=n1d) < 0.3
+    systems, seed = [], 0
+    for i, N in enumerate(Ns):
+        pot = ["Cuspy", "SemiSmooth", "Smooth"][i % 3]
+        kw = dict(PARAMS, k_frame=1.0 / N, shape=[int(N)], seed=seed, nchunk=nchunk)
+        if quartic[i]:
+            kw.pop("k_interactions")
+            kw.update(a1=1.0, a2=5.0)
+        if pot == "SemiSmooth":
+            kw["kappa"] = 1.0
+        systems.append((f"System_{pot}_{'Quartic' if quartic[i] else 'Laplace'}", kw))
+        seed += int(N)
+    for _ in range(n2d):
+        N = side * side
+        systems.append(
+            ("Line2d.System_Cuspy_Laplace", dict(PARAMS, k_frame=1.0 / N, shape=[side, side], seed=seed, nchunk=nchunk))
+        )
+        seed += N
+    return systems
+
+
+def ragged_nopassing(systems, max_n=8192):
+    """No-passing versions of the Cuspy-Laplace chains of the ragged workload."""
+    out = []
+    for name, kw in systems:
+        if name == "System_Cuspy_Laplace" and kw["shape"][0] <= max_n:
+            out.append(("System_Cuspy_Laplace_Nopassing", dict(kw, eta=0.0, dt=0.0)))
+    return out
+
+
+def rg_cpu_baseline(target_s=10.0):
+    """The oracle on a sample of the ragged systems (one per thread): events of
+    the same protocol, counted as sum N x time steps."""
+    from oracle import pyoracle
+
+    pyoracle.lib()
+    cores = min(16, os.cpu_count() or 1)
+    systems = ragged_systems()
+    rng = np.random.default_rng(1)
+    pick = rng.choice(len(systems) - 8, size=cores, replace=False)
+    cls = {"Cuspy": "Cuspy", "SemiSmooth": "SemiSmooth", "Smooth": "Smooth"}
+    objs = []
+    for i in pick:
+        name, kw = systems[i]
+        pot = name.split("_")[1]
+        inter = "Quartic1d" if name.endswith("Quartic") else "Laplace1d"
+        s = pyoracle._System(rank=1, potential=cls[pot], interactions=inter, **kw)
+        s.minimise()
+        objs.append((s, pot == "Smooth", int(np.prod(kw["shape"]))))
+
+    def event(item):
+        s, smooth, N = item
+        i0 = s.inc
+        if not smooth:
+            s.eventDrivenStep(RG_EPS, False)
+        s.eventDrivenStep(RG_EPS, True)
+        s.minimise()
+        return N * (s.inc - i0)
+
+    work, t0 = 0, time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        while time.perf_counter() - t0 < target_s:
+            work += sum(ex.map(event, objs))
+    dt = time.perf_counter() - t0
+    return {
+        "value": work / dt,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{cores} randomly chosen systems of the workload (sizes {sorted(o[2] for o in objs)}), "
+        f"one per thread, events of the same protocol for {dt:.1f} s",
+    }
+
+
+def run_ragged(args):
+    import torch
+
+    from frictionqpotspringblock_amd.ragged import Ragged
+
+    torch.cuda.init()
+    systems = ragged_systems()
+    rg = Ragged(systems)
+    sizes = np.array(rg.sizes, dtype=float)
+    smooth = np.array(["Smooth" in n for n, _ in systems])
+    nps = ragged_nopassing(systems)
+    rgn = Ragged(nps)
+    nsizes = np.array(rgn.sizes, dtype=float)
+    rg.minimise()
+    rgn.minimise()
+    for _ in range(args.warmup):
+        rg.eventDrivenStep(RG_EPS, False, active=~smooth)
+        rg.eventDrivenStep(RG_EPS, True)
+        rg.minimise()
+    torch.cuda.synchronize()
+    work = 0.0
+    events = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rg.eventDrivenStep(RG_EPS, False, active=~smooth)
+        rg.eventDrivenStep(RG_EPS, True)
+        rg.minimise()
+        work += float(np.sum(sizes * rg.last_steps))
+        events += 1
+    t_inertial = time.perf_counter() - t0
+    for _ in range(args.steps):
+        rgn.eventDrivenStep(RG_EPS, False)
+        rgn.eventDrivenStep(RG_EPS, True)
+        rgn.minimise()
+        work += float(np.sum(nsizes * rgn.last_steps))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cpu = None if args.no_cpu else rg_cpu_baseline()
+    return {
+        "metric": METRIC,
+        "value": work / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Weibull yield landscapes from pcg32, disjoint seeds per system)",
+        "config": {
+            "workload": f"ragged: {len(systems)} systems ({int(sizes.sum())} particles, {rg.groups} launch groups), "
+            f"{args.steps} events each, then {args.steps} no-passing events on {len(nps)} Cuspy-Laplace chains",
+            "systems": len(systems),
+            "particles": int(sizes.sum()),
+            "groups": rg.groups,
+            "nopassing_systems": len(nps),
+            "inertial_seconds": t_inertial,
+            "particle_steps": work,
+        },
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
 def run_flow(args):
     import torch
 
@@ -444,11 +599,12 @@ def run_flow(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["flow", "longrange", "adjoint"], default="flow")
+    ap.add_argument("--workload", choices=["flow", "longrange", "adjoint", "ragged"], default="flow")
     ap.add_argument(
         "--steps", type=int, default=None,
         help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
-        "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5)",
+        "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5; "
+        "ragged: quasistatic events, default 20)",
     )
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -477,8 +633,10 @@ def main():
         )
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5}[args.workload]
-    run = {"flow": run_flow, "longrange": run_longrange, "adjoint": run_adjoint}[args.workload]
+        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 20}[args.workload]
+    run = {"flow": run_flow, "longrange": run_longrange, "adjoint": run_adjoint, "ragged": run_ragged}[
+        args.workload
+    ]
     line = run(args)
     print(json.dumps(line), flush=True)
 

@@ -192,6 +192,7 @@ SIGNATURES = {
     "fqsb_ragged_set": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_ragged_get_u_frame": (C.c_int, [_P, _dp]),
     "fqsb_ragged_get_inc": (C.c_int, [_P, _ip]),
+    "fqsb_ragged_last_steps": (C.c_int, [_P, _up]),
     "fqsb_ragged_chunk_index_at_align": (C.c_int, [_P, _ip]),
 }
 

@@ -1492,12 +1492,14 @@ static std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol,
     }
     g.reset_run(fq::MODE_MINIMISE);
     g.push();
+    // particles per thread: the fewest for which the launch fits the kernel's
+    // thread bound (1024 up to 8 particles per thread, 512 at 16)
     int ppt = 1;
-    while ((g.N + ppt - 1) / ppt > 1024) {
+    while (ppt <= 16 && (g.N + ppt - 1) / ppt > (ppt <= 8 ? 1024 : 512)) {
         ppt *= 2;
     }
     if (ppt > 16) {
-        throw Error("system too large for the no-passing kernel");
+        throw Error("system too large for the no-passing kernel (at most 8192 particles)");
     }
     const int64_t need = (g.N + ppt - 1) / ppt;
     const int threads = (int)(((need + 63) / 64) * 64);
@@ -2326,6 +2328,14 @@ int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out)
 {
     for (size_t i = 0; i < r->where.size(); ++i) {
         out[i] = r->groups[r->where[i].first]->hrs[(size_t)r->where[i].second].u_frame;
+    }
+    return 0;
+}
+
+int fqsb_ragged_last_steps(fqsb_ragged* r, uint64_t* out)
+{
+    for (size_t i = 0; i < r->where.size(); ++i) {
+        out[i] = (uint64_t)r->groups[r->where[i].first]->hrs[(size_t)r->where[i].second].steps;
     }
     return 0;
 }

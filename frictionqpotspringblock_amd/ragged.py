@@ -153,6 +153,13 @@ class Ragged:
         return out
 
     @property
+    def last_steps(self):
+        """Steps (or no-passing sweeps) each system took in the last run."""
+        out = np.empty(self._n, dtype=np.uint64)
+        _check(capi().fqsb_ragged_last_steps(self._h, out.ctypes.data_as(_up)))
+        return out
+
+    @property
     def index_at_align(self):
         out = np.empty(self._offsets[-1], dtype=np.int64)
         _check(capi().fqsb_ragged_chunk_index_at_align(self._h, out.ctypes.data_as(_ip)))

@@ -312,6 +312,9 @@ int fqsb_ragged_get(fqsb_ragged* r, int field, double* out);
 int fqsb_ragged_set(fqsb_ragged* r, int field, const double* in);
 int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out);
 int fqsb_ragged_get_inc(fqsb_ragged* r, int64_t* out);
+/* Steps (time steps, or no-passing sweeps) each system took in the last
+ * minimise / timeSteps / flowSteps call ([n]). */
+int fqsb_ragged_last_steps(fqsb_ragged* r, uint64_t* out);
 int fqsb_ragged_chunk_index_at_align(fqsb_ragged* r, int64_t* out);
 
 #ifdef __cplusplus

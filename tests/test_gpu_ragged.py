@@ -160,3 +160,9 @@ def test_ragged_masks_and_errors(oracle):
         rg.minimise(max_iter=3, active=~smooth)
     with pytest.raises(RuntimeError, match="unknown system class"):
         Ragged([("System_Nothing", _par(10, 0))])
+    # the no-passing kernel holds up to 8192 particles (16 per thread, 512 threads)
+    big = Ragged([("System_Cuspy_Laplace_Nopassing", dict(_par(9000, 0), eta=0.0, dt=0.0))])
+    with pytest.raises(RuntimeError, match="too large for the no-passing kernel"):
+        big.minimise()
+    ok = Ragged([("System_Cuspy_Laplace_Nopassing", dict(_par(8192, 0), eta=0.0, dt=0.0))])
+    assert ok.minimise()[0] == 0
