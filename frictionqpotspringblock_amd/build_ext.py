@@ -47,6 +47,7 @@ HIP_SOURCES = [
     "fq_stream.hip",
     "fq_longrange.hip",
     "fq_adjoint.hip",
+    "fq_tile.hip",
     "fq_res_cuspy_laplace.hip",
     "fq_res_cuspy_chain.hip",
     "fq_res_cuspy_field.hip",

@@ -86,6 +86,8 @@ constexpr uint64_t MIN_BLOCK = 1u << 18;
 constexpr uint64_t STEP_BLOCK = 1u << 16;
 // steps between host checks of a batched long-range run (matrix form)
 constexpr uint64_t LR_BLOCK = 128;
+// steps between host checks of a tiled run
+constexpr uint64_t TILE_BLOCK = 256;
 
 struct Group {
     fqsb_desc d{};
@@ -129,6 +131,15 @@ struct Group {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int ppt = 0, maxt = 0, threads = 0;
     bool streaming = false; // systems too large for a resident workgroup
+    // tiled per-step path for streaming groups (fq_tile.hpp); FQSB_NO_TILES=1
+    // keeps them on the one-workgroup streaming kernel
+    bool tiled = false;
+    int ntiles = 0;
+    size_t tile_lds = 0;
+    fq::TileDesc* tiles = nullptr;
+    int32_t* tile_first = nullptr;
+    double* tile_part = nullptr;
+    int32_t* tile_flag = nullptr;
     // FQSB_GENERAL_KERNEL=1: timeSteps / flowSteps on k_resident instead of
     // k_flow (the two give the same results; a switch for comparisons)
     bool general_kernel = false;
@@ -184,6 +195,10 @@ struct Group {
         dfree(lr_un);
         dfree(lr_part);
         free_adjoint();
+        dfree(tiles);
+        dfree(tile_first);
+        dfree(tile_part);
+        dfree(tile_flag);
         dfree(lr_yl);
         dfree(lr_yr);
         if (ev0) {
@@ -386,8 +401,92 @@ struct Group {
     // launches of at most `block` steps. A launch that stops before a step
     // needing a landscape refill (ST_REFILL) is followed by k_refill and a
     // relaunch. Returns the time between events around all launches (ms).
+    // Systems over many workgroups: per step one k_tile_step and one
+    // k_tile_decide for all systems, checked by the host every `block` steps;
+    // a pending refill normalises the state (k_tile_finish), refills
+    // (k_refill) and refreshes the cached wells (k_tile_prologue).
+    float run_tiled(fq::RunArgs ra, uint64_t total, uint64_t block, bool timed)
+    {
+        reset_run(ra.mode);
+        for (auto& r : hrs) {
+            r.target = (int64_t)total;
+            r.cur = 0;
+        }
+        push();
+        if (timed) {
+            HIPC(hipEventRecord(ev0, stream));
+        }
+        HIPC(fq::launch_tile_prologue(views, tiles, ntiles, stream));
+        auto normalise = [&] {
+            if (!fq::launch_tile_finish(pot, inter, views, tiles, ntiles, stream)) {
+                throw Error("no tiled kernel for this potential / interaction");
+            }
+            HIPC(hipGetLastError());
+            pull();
+            for (auto& r : hrs) {
+                r.cur = 0;
+            }
+            push();
+        };
+        uint64_t todo = total;
+        while (true) {
+            const uint64_t nb = std::min<uint64_t>(block, std::max<uint64_t>(todo, 1));
+            for (uint64_t s = 0; s < nb; ++s) {
+                if (!fq::launch_tile_step(pot, inter, views, tiles, ntiles, tile_lds, tile_part, tile_flag, ra,
+                                          stream)) {
+                    throw Error("no tiled kernel for this potential / interaction");
+                }
+                HIPC(hipGetLastError());
+                HIPC(fq::launch_tile_decide(views, (int)R, tile_first, tile_part, tile_flag, ra, stream));
+            }
+            pull();
+            bool refill = false, more = false;
+            int64_t lag = 0;
+            for (const auto& r : hrs) {
+                refill |= r.status == fq::ST_REFILL;
+                if ((r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET || r.status == fq::ST_REFILL) &&
+                    r.steps < r.target) {
+                    more |= r.status != fq::ST_REFILL;
+                    lag = std::max<int64_t>(lag, r.target - r.steps);
+                }
+            }
+            todo = (uint64_t)lag;
+            if (refill) {
+                normalise();
+                HIPC(fq::launch_refill(views, (int)R, stream));
+                ++refills;
+                for (auto& r : hrs) {
+                    if (r.status == fq::ST_REFILL) {
+                        r.status = fq::ST_RUNNING;
+                    }
+                }
+                HIPC(fq::launch_tile_prologue(views, tiles, ntiles, stream));
+                continue;
+            }
+            if (!more) {
+                break;
+            }
+        }
+        normalise();
+        float ms = 0.0f;
+        if (timed) {
+            HIPC(hipEventRecord(ev1, stream));
+            HIPC(hipEventSynchronize(ev1));
+            HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        }
+        for (auto& r : hrs) {
+            if (r.status == fq::ST_RUNNING) {
+                r.status = fq::ST_BUDGET;
+            }
+        }
+        return ms;
+    }
+
     float run(fq::RunArgs ra, uint64_t total, uint64_t block, bool timed)
     {
+        if (tiled && (ra.mode == fq::MODE_STEPS || (ra.mode == fq::MODE_MINIMISE && !ra.time_activity))) {
+            return run_tiled(ra, total, std::min<uint64_t>(block, TILE_BLOCK), timed);
+        }
         reset_run(ra.mode);
         for (auto& r : hrs) {
             r.target = (int64_t)total;
@@ -1079,6 +1178,55 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds)
             g->yl2 = dalloc<double>(RN);
             g->yr2 = dalloc<double>(RN);
             g->jl2 = dalloc<int32_t>(RN);
+        }
+    }
+
+    // tiles of the streaming systems (chains: runs of particles; lattices:
+    // whole rows, at most rows - 2 per tile so that the halo rows are distinct)
+    if (g->streaming && fq::tile_supported(g->pot, inter, g->ext)) {
+        const char* nt = std::getenv("FQSB_NO_TILES");
+        bool ok = !(nt && nt[0] == '1');
+        std::vector<fq::TileDesc> tl;
+        std::vector<int32_t> first((size_t)R + 1, 0);
+        size_t lds = 0;
+        for (int64_t r = 0; r < R && ok; ++r) {
+            const fqsb_desc& d = ds[(size_t)r];
+            const int64_t N = Ns[(size_t)r];
+            const bool lat = inter == FQSB_INT_LAPLACE2D || inter == FQSB_INT_QUARTICGRADIENT2D;
+            const int64_t cols = d.rank == 1 ? 1 : (int64_t)d.cols;
+            const int64_t rows = (int64_t)d.rows;
+            int64_t step;
+            if (lat) {
+                int64_t rpt = std::max<int64_t>(1, 4096 / cols);
+                rpt = std::min<int64_t>(rpt, rows - 2);
+                ok = rows >= 3 && rpt >= 1;
+                step = rpt * cols;
+            }
+            else {
+                ok = N >= 3;
+                step = 4096;
+            }
+            first[(size_t)r] = (int32_t)tl.size();
+            for (int64_t p0 = 0; ok && p0 < N; p0 += step) {
+                fq::TileDesc td{};
+                td.sys = (int32_t)r;
+                td.p0 = p0;
+                td.p1 = std::min<int64_t>(N, p0 + step);
+                tl.push_back(td);
+                lds = std::max<size_t>(lds, sizeof(double) * (size_t)((td.p1 - td.p0) + 2 * (lat ? cols : 1)));
+            }
+        }
+        first[(size_t)R] = (int32_t)tl.size();
+        if (ok && lds <= g->lds_limit && !tl.empty()) {
+            g->tiled = true;
+            g->ntiles = (int)tl.size();
+            g->tile_lds = lds;
+            g->tiles = dalloc<fq::TileDesc>(tl.size());
+            g->h2d(g->tiles, tl.data(), tl.size());
+            g->tile_first = dalloc<int32_t>((size_t)R + 1);
+            g->h2d(g->tile_first, first.data(), (size_t)R + 1);
+            g->tile_part = dalloc<double>(2 * tl.size());
+            g->tile_flag = dalloc<int32_t>((size_t)R);
         }
     }
 

@@ -101,7 +101,7 @@ struct RunState {
     int32_t below;  // trailing count of residuals < tol
     int32_t below2; // trailing count of residuals < tol^2
     int32_t dec;    // trailing count of strictly decreasing pairs
-    int32_t pad;
+    int32_t cur;    // tiled path: the state buffer (0: u, v, a; 1: u_b, v_b, a_b) holding the state
     double prev;    // last residual (+inf initially)
     int64_t s_n;    // activity sum of the previous step
     int64_t qs_first, qs_last;
@@ -140,6 +140,14 @@ struct SysView {
 };
 
 // Uniform arguments of one resident launch.
+// One tile of the tiled per-step path (fq_tile.hpp): particles [p0, p1) of
+// system `sys` (whole rows for lattices).
+struct TileDesc {
+    int32_t sys;
+    int32_t pad;
+    int64_t p0, p1;
+};
+
 // Extra device pointers of a batched long-range group (all [R, N], row r =
 // replica r), used by the matrix-core form of LongRange1d.
 struct LrBuffers {

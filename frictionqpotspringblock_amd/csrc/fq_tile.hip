@@ -1,0 +1,69 @@
+// Launchers of the tiled per-step path (fq_tile.hpp).
+#include "fq_launch.hpp"
+#include "fq_tile.hpp"
+
+namespace fq {
+
+#define FQ_TILE_INTS(X, P)                                                                                     \
+    X(P, INT_NONE) X(P, INT_LAPLACE1D) X(P, INT_QUARTIC1D) X(P, INT_QUARTICGRADIENT1D) X(P, INT_LAPLACE2D)    \
+        X(P, INT_QUARTICGRADIENT2D)
+
+bool tile_supported(int pot, int inter, int ext)
+{
+    (void)pot;
+    return ext == 0 && (inter == INT_NONE || inter == INT_LAPLACE1D || inter == INT_QUARTIC1D ||
+                        inter == INT_QUARTICGRADIENT1D || inter == INT_LAPLACE2D ||
+                        inter == INT_QUARTICGRADIENT2D);
+}
+
+hipError_t launch_tile_prologue(const SysView* views, const TileDesc* tiles, int ntiles, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_tile_prologue, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
+    return hipGetLastError();
+}
+
+bool launch_tile_step(int pot, int inter, const SysView* views, const TileDesc* tiles, int ntiles, size_t lds,
+                      double* part, int32_t* tflag, const RunArgs& ra, hipStream_t stream)
+{
+#define FQ_STEP(P, I)                                                                                          \
+    if (pot == P && inter == I) {                                                                              \
+        hipLaunchKernelGGL((k_tile_step<P, I>), dim3((unsigned)ntiles), dim3(TILE_THREADS), lds, stream, views, \
+                           tiles, part, tflag, ra);                                                            \
+        return true;                                                                                           \
+    }
+    FQ_TILE_INTS(FQ_STEP, POT_CUSPY)
+    FQ_TILE_INTS(FQ_STEP, POT_SEMISMOOTH)
+    FQ_TILE_INTS(FQ_STEP, POT_SMOOTH)
+#undef FQ_STEP
+    return false;
+}
+
+hipError_t launch_tile_decide(const SysView* views, int nsys, const int32_t* tile_first, const double* part,
+                              int32_t* tflag, const RunArgs& ra, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_tile_decide, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, stream, views, nsys,
+                       tile_first, part, tflag, ra);
+    return hipGetLastError();
+}
+
+bool launch_tile_finish(int pot, int inter, const SysView* views, const TileDesc* tiles, int ntiles,
+                        hipStream_t stream)
+{
+    bool ok = false;
+#define FQ_FIN(P, I)                                                                                           \
+    if (!ok && pot == P && inter == I) {                                                                       \
+        hipLaunchKernelGGL((k_tile_finish<P, I>), dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, \
+                           tiles);                                                                             \
+        ok = true;                                                                                             \
+    }
+    FQ_TILE_INTS(FQ_FIN, POT_CUSPY)
+    FQ_TILE_INTS(FQ_FIN, POT_SEMISMOOTH)
+    FQ_TILE_INTS(FQ_FIN, POT_SMOOTH)
+#undef FQ_FIN
+    if (ok) {
+        hipLaunchKernelGGL(k_tile_finish_u, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
+    }
+    return ok;
+}
+
+} // namespace fq

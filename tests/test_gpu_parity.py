@@ -25,14 +25,19 @@ def fq():
     return fq
 
 
-@pytest.fixture(params=["resident", "streaming"])
+@pytest.fixture(params=["resident", "tiled", "streaming"])
 def path(request, monkeypatch):
-    """Run a test through the resident kernels and again through the streaming
-    kernel (selected for every size by FQSB_FORCE_STREAMING=1)."""
-    if request.param == "streaming":
+    """Run a test through the resident kernels, again with every system on
+    the large-system path (FQSB_FORCE_STREAMING=1: the tiled per-step kernels
+    for timeSteps / flowSteps / minimise, the one-workgroup streaming kernel
+    otherwise), and again with the tiles switched off (FQSB_NO_TILES=1: the
+    streaming kernel for everything)."""
+    monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
+    monkeypatch.delenv("FQSB_NO_TILES", raising=False)
+    if request.param in ("tiled", "streaming"):
         monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
-    else:
-        monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
+    if request.param == "streaming":
+        monkeypatch.setenv("FQSB_NO_TILES", "1")
     return request.param
 
 
