@@ -140,6 +140,15 @@ struct Group {
     int32_t* tile_first = nullptr;
     double* tile_part = nullptr;
     int32_t* tile_flag = nullptr;
+    // captured blocks of tiled steps (hipGraph), keyed by the run arguments and
+    // the number of steps; FQSB_NO_GRAPHS=1 launches the kernels one by one
+    struct TileGraph {
+        fq::RunArgs ra;
+        uint64_t steps;
+        hipGraphExec_t exec;
+    };
+    std::vector<TileGraph> tile_graphs;
+    bool use_graphs = true;
     // FQSB_GENERAL_KERNEL=1: timeSteps / flowSteps on k_resident instead of
     // k_flow (the two give the same results; a switch for comparisons)
     bool general_kernel = false;
@@ -195,6 +204,9 @@ struct Group {
         dfree(lr_un);
         dfree(lr_part);
         free_adjoint();
+        for (auto& tg : tile_graphs) {
+            (void)hipGraphExecDestroy(tg.exec);
+        }
         dfree(tiles);
         dfree(tile_first);
         dfree(tile_part);
@@ -428,9 +440,7 @@ struct Group {
             }
             push();
         };
-        uint64_t todo = total;
-        while (true) {
-            const uint64_t nb = std::min<uint64_t>(block, std::max<uint64_t>(todo, 1));
+        auto enqueue = [&](uint64_t nb) {
             for (uint64_t s = 0; s < nb; ++s) {
                 if (!fq::launch_tile_step(pot, inter, views, tiles, ntiles, tile_lds, tile_part, tile_flag, ra,
                                           stream)) {
@@ -439,6 +449,49 @@ struct Group {
                 HIPC(hipGetLastError());
                 HIPC(fq::launch_tile_decide(views, (int)R, tile_first, tile_part, tile_flag, ra, stream));
             }
+        };
+        // a block of nb steps as one graph launch (captured once per run
+        // arguments and length; the kernels read the run state from device
+        // memory, so a replay continues where the last one stopped)
+        auto launch_block = [&](uint64_t nb) {
+            if (!use_graphs) {
+                enqueue(nb);
+                return;
+            }
+            for (const auto& tg : tile_graphs) {
+                if (tg.steps == nb && std::memcmp(&tg.ra, &ra, sizeof(ra)) == 0) {
+                    HIPC(hipGraphLaunch(tg.exec, stream));
+                    return;
+                }
+            }
+            hipGraph_t graph = nullptr;
+            HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(nb);
+            }
+            catch (...) {
+                (void)hipStreamEndCapture(stream, &graph);
+                if (graph) {
+                    (void)hipGraphDestroy(graph);
+                }
+                throw;
+            }
+            HIPC(hipStreamEndCapture(stream, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPC(e);
+            if (tile_graphs.size() >= 8) {
+                (void)hipGraphExecDestroy(tile_graphs.front().exec);
+                tile_graphs.erase(tile_graphs.begin());
+            }
+            tile_graphs.push_back(TileGraph{ra, nb, exec});
+            HIPC(hipGraphLaunch(exec, stream));
+        };
+        uint64_t todo = total;
+        while (true) {
+            const uint64_t nb = std::min<uint64_t>(block, std::max<uint64_t>(todo, 1));
+            launch_block(nb);
             pull();
             bool refill = false, more = false;
             int64_t lag = 0;
@@ -1114,6 +1167,9 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds)
     if (const char* gk = std::getenv("FQSB_GENERAL_KERNEL")) {
         g->general_kernel = gk[0] == '1';
     }
+    if (const char* ng = std::getenv("FQSB_NO_GRAPHS")) {
+        g->use_graphs = ng[0] != '1';
+    }
     HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIPC(hipEventCreate(&g->ev0));
     HIPC(hipEventCreate(&g->ev1));
@@ -1382,9 +1438,20 @@ struct fqsb_ragged {
     std::vector<int64_t> sizes;                    // N of system i
     std::vector<int64_t> uoff;                     // first element of system i in concatenated arrays
 
+    // FQSB_RAGGED_SERIAL=1: run the groups one after the other from the
+    // calling thread (for tools that do not follow launches from many threads)
     template <class F>
     void each(F&& fn)
     {
+        if (const char* sr = std::getenv("FQSB_RAGGED_SERIAL")) {
+            if (sr[0] == '1') {
+                for (size_t gi = 0; gi < groups.size(); ++gi) {
+                    HIPC(hipSetDevice(groups[gi]->device));
+                    fn(*groups[gi], gi);
+                }
+                return;
+            }
+        }
         std::vector<std::exception_ptr> err(groups.size());
         std::vector<std::thread> th;
         th.reserve(groups.size());

@@ -389,3 +389,25 @@ def test_large_systems(fq, oracle, sub, cls, n, extra, minimise):
         assert a.minimise(time_activity=True) == b.minimise(time_activity=True) == 0
         assert a.quasistaticActivityFirst == b.quasistaticActivityFirst
         assert_same_state(a, b, "kick + minimise")
+
+
+def test_tiled_graphs_match_eager(fq, monkeypatch):
+    """Blocks of tiled steps replayed from captured graphs give the results of
+    the kernels launched one by one, call after call (with host work and
+    synchronisations between the replays)."""
+    par = _base(20000, k_interactions=1.0, nchunk=60)
+    monkeypatch.setenv("FQSB_NO_GRAPHS", "1")
+    eager = fq.Line1d.System_Cuspy_Laplace(**par)
+    monkeypatch.delenv("FQSB_NO_GRAPHS")
+    graph = fq.Line1d.System_Cuspy_Laplace(**par)
+    v0 = np.random.default_rng(3).normal(0.0, 5.0, 20000)
+    for s in (eager, graph):
+        s.v = v0
+    for rep in range(3):
+        for s in (eager, graph):
+            s.flowSteps(700, 0.05)  # blocks of 256, 256, 188 steps: two graphs, replayed
+        assert_same_state(eager, graph, f"flowSteps {rep}")
+    for s in (eager, graph):
+        s.eventDrivenStep(1e-3, True)
+        assert s.minimise() == 0
+    assert_same_state(eager, graph, "minimise")
