@@ -411,3 +411,35 @@ def test_tiled_graphs_match_eager(fq, monkeypatch):
         s.eventDrivenStep(1e-3, True)
         assert s.minimise() == 0
     assert_same_state(eager, graph, "minimise")
+
+
+def test_c0_quasistatic_protocol(fq, oracle):
+    """The reference's CPU configuration: a 1024-particle Cuspy-Laplace chain
+    (Weibull k = 2 yield increments, offset -100, chunks of 200 wells), 100
+    quasistatic events of the examples' protocol (examples/Line1d_Cuspy_
+    Laplace.py:39-61). Avalanche sizes, increments and frame positions are
+    exact; the final state is bit-identical."""
+    par = dict(
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) * 0.1,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / 1024,
+        dt=0.1,
+        shape=[1024],
+        seed=0,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=200,
+    )
+    a = fq.Line1d.System_Cuspy_Laplace(**par)
+    b = oracle.Line1d.System_Cuspy_Laplace(**par)
+    ua, fa, Sa, ia = P.quasistatic(a, 201)
+    ub, fb, Sb, ib = P.quasistatic(b, 201)
+    assert np.array_equal(Sa, Sb)
+    assert np.array_equal(ia, ib)
+    assert np.array_equal(ua, ub)
+    np.testing.assert_allclose(fa, fb, rtol=1e-12, atol=1e-14)
+    assert np.sum(np.abs(Sa)) > 0
+    assert_same_state(a, b, "after 100 events")
