@@ -157,6 +157,14 @@ SIGNATURES = {
     "fqsb_batch_time_steps": (C.c_int, [_P, C.c_uint64, C.c_uint64, _fp]),
     "fqsb_batch_minimise": (C.c_int, [_P, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up]),
     "fqsb_batch_event_driven_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _dp]),
+    "fqsb_batch_quasistatic": (
+        C.c_int,
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp, _ip],
+    ),
+    "fqsb_quasistatic": (
+        C.c_int,
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp, _ip],
+    ),
     "fqsb_batch_get": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_batch_set": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_batch_get_u_frame": (C.c_int, [_P, _dp]),
@@ -188,6 +196,11 @@ SIGNATURES = {
         C.c_int,
         [_P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_uint8), _dp],
     ),
+    "fqsb_ragged_quasistatic": (
+        C.c_int,
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint8),
+         _ip, _dp, _dp, _ip],
+    ),
     "fqsb_ragged_get": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_ragged_set": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_ragged_get_u_frame": (C.c_int, [_P, _dp]),
@@ -212,6 +225,31 @@ def capi():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def _quasistatic(call, nsys, eps, nevents, first_event, direction, tol, niter_tol, max_iter, masked=False,
+                 mask=None):
+    """Shared wrapper of fqsb_*_quasistatic: outputs [nsys, nevents]; `masked`:
+    the entry point takes an `active` mask (None: every system)."""
+    nevents = int(nevents)
+    n = max(nevents, 0)
+    out = dict(
+        S=np.zeros([nsys, n], dtype=np.int64),
+        u_frame=np.zeros([nsys, n]),
+        f_frame=np.zeros([nsys, n]),
+        inc=np.zeros([nsys, n], dtype=np.int64),
+    )
+    args = [int(first_event), nevents, float(eps), int(direction), float(tol), int(niter_tol), int(max_iter)]
+    if masked:
+        args.append(None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+    args += [
+        out["S"].ctypes.data_as(_ip),
+        out["u_frame"].ctypes.data_as(_dp),
+        out["f_frame"].ctypes.data_as(_dp),
+        out["inc"].ctypes.data_as(_ip),
+    ]
+    _check(call(*args))
+    return out
 
 
 def _check(rc):
@@ -426,6 +464,19 @@ class Batch:
             )
         )
         return ret
+
+    def quasistatic(self, eps, nevents, first_event=0, direction=1, tol=1e-5, niter_tol=10, max_iter=int(1e9)):
+        """The examples' quasistatic loop (examples/Line1d_Cuspy_Laplace.py:39-61)
+        for every replica, each at its own pace on the device: events
+        first_event .. first_event + nevents - 1 (0: u_frame = 0 and minimise;
+        odd: eventDrivenStep(eps, False); even: eventDrivenStep(eps, True) and
+        minimise). Returns a dict of [R, nevents] arrays: "S" (sum of the
+        changes of chunk.index_at_align), "u_frame", "f_frame" (mean frame
+        force) and "inc" after each event."""
+        return _quasistatic(
+            lambda *a: capi().fqsb_batch_quasistatic(self._h, *a), self._R, eps, nevents, first_event, direction,
+            tol, niter_tol, max_iter,
+        )
 
     def _get(self, name):
         out = np.empty(self._R * self._N)

@@ -89,6 +89,9 @@ constexpr uint64_t LR_BLOCK = 128;
 // steps between host checks of a tiled run
 constexpr uint64_t TILE_BLOCK = 256;
 
+struct Group;
+std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter);
+
 struct Group {
     fqsb_desc d{};
     int64_t R = 0, rows = 0, cols = 0, nchunk = 0;
@@ -1053,6 +1056,230 @@ struct Group {
         }
         return ret;
     }
+
+    // The quasistatic protocol of the reference's examples
+    // (examples/Line1d_Cuspy_Laplace.py:39-61) for every system (with mask
+    // set: the masked-in ones): events k = first .. first + nevents - 1,
+    //   k == 0:      u_frame = 0, minimise
+    //   k odd:       eventDrivenStep(eps, false, direction)
+    //   k even > 0:  eventDrivenStep(eps, true, direction), minimise
+    // recording after event e, at [r * nevents + e] of the host arrays (any
+    // may be null): S (sum of well-index changes), u_frame, mean f_frame, inc.
+    // Systems that fit a resident workgroup run the whole protocol on the
+    // device (k_quasi, one launch per step budget, refill or end); the others
+    // are driven event by event from here. Returns the time around the device
+    // work (ms).
+    float quasistatic(int64_t first, int64_t nevents, double eps, int direction, double tol, uint64_t niter_tol,
+                      uint64_t max_iter, int64_t* S, double* uf, double* ff, int64_t* inc_out)
+    {
+        if (first < 0 || nevents < 0) {
+            throw Error("the first event and the number of events must be non-negative");
+        }
+        if (!(eps > 0.0) || !std::isfinite(eps)) {
+            throw Error("eps must be positive and finite");
+        }
+        if (direction != 1 && direction != -1) {
+            throw Error("direction must be +1 or -1");
+        }
+        if (!(tol < 1.0) || max_iter < 1) {
+            throw Error("minimise needs tol < 1 and max_iter >= 1");
+        }
+        if (minim == FQSB_MIN_NONE) {
+            throw Error("Minimisation not implementated");
+        }
+        if (pot == FQSB_POT_SMOOTH && nevents > (first == 0 ? 1 : 0)) {
+            throw Error("Operation not possible.");
+        }
+        if (ext) {
+            throw Error("eventDrivenStep is not available for this system");
+        }
+        if (nevents == 0) {
+            return 0.0f;
+        }
+        const size_t n = (size_t)(R * nevents);
+        int64_t* dS = dalloc<int64_t>(n);
+        double* duf = dalloc<double>(n);
+        double* dff = dalloc<double>(n);
+        int64_t* dinc = dalloc<int64_t>(n);
+        fq::QsArgs qa{};
+        qa.first = first;
+        qa.nevents = nevents;
+        qa.eps = eps;
+        qa.direction = direction;
+        qa.max_iter = (int64_t)std::min<uint64_t>(max_iter, (uint64_t)std::numeric_limits<int64_t>::max());
+        qa.out_S = dS;
+        qa.out_uf = duf;
+        qa.out_ff = dff;
+        qa.out_inc = dinc;
+        float ms = 0.0f;
+        try {
+            set_iref_current();
+            const bool device = !streaming && minim == FQSB_MIN_INERTIAL &&
+                                std::getenv("FQSB_HOST_QUASISTATIC") == nullptr;
+            if (!device || !quasistatic_device(qa, tol, niter_tol, ms)) {
+                ms = quasistatic_host(qa, tol, niter_tol, max_iter);
+            }
+            auto fetch = [&](auto* host, const auto* dev) {
+                if (host) {
+                    d2h(host, dev, n);
+                }
+            };
+            fetch(S, dS);
+            fetch(uf, duf);
+            fetch(ff, dff);
+            fetch(inc_out, dinc);
+        }
+        catch (...) {
+            (void)hipStreamSynchronize(stream);
+            dfree(dS);
+            dfree(duf);
+            dfree(dff);
+            dfree(dinc);
+            throw;
+        }
+        dfree(dS);
+        dfree(duf);
+        dfree(dff);
+        dfree(dinc);
+        return ms;
+    }
+
+    // k_quasi for the whole protocol; false if it is not instantiated for this
+    // group (the caller then drives the events from the host)
+    bool quasistatic_device(const fq::QsArgs& qa, double tol, uint64_t niter_tol, float& ms)
+    {
+        for (size_t i = 0; i < hrs.size(); ++i) {
+            auto& r = hrs[i];
+            r.status = (mask && !mask[i]) ? fq::ST_IDLE : fq::ST_RUNNING;
+            r.steps = 0;
+            r.below = 0;
+            r.below2 = 0;
+            r.dec = 0;
+            r.prev = std::numeric_limits<double>::infinity();
+            r.qs_event = 0;
+            r.qs_phase = fq::QS_BEGIN;
+            r.qs_isum = 0;
+            r.refill_u = 0;
+        }
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_MINIMISE;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        // steps per launch (FQSB_QS_BLOCK: another budget, e.g. to exercise relaunches)
+        ra.max_steps = (int64_t)MIN_BLOCK;
+        if (const char* e = std::getenv("FQSB_QS_BLOCK")) {
+            ra.max_steps = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), (int64_t)MIN_BLOCK));
+        }
+        const size_t lds = fq::resident_lds(inter, N, threads);
+        push();
+        HIPC(hipEventRecord(ev0, stream));
+        for (bool first_launch = true;; first_launch = false) {
+            if (!fq::launch_quasi(pot, inter, ppt, maxt, threads, lds, views, (int)R, ra, qa, stream)) {
+                if (first_launch) {
+                    return false;
+                }
+                throw Error("quasistatic kernel not available");
+            }
+            HIPC(hipGetLastError());
+            pull();
+            bool refill = false, more = false;
+            for (const auto& r : hrs) {
+                if (r.status == fq::ST_NAN) {
+                    throw Error("NaN entries found");
+                }
+                if (r.status == fq::ST_MAXITER) {
+                    throw Error("No convergence found");
+                }
+                refill |= r.status == fq::ST_REFILL;
+                more |= r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET;
+            }
+            if (refill) {
+                HIPC(fq::launch_refill(views, (int)R, stream));
+                ++refills;
+                for (auto& r : hrs) {
+                    if (r.status == fq::ST_REFILL) {
+                        r.status = fq::ST_RUNNING;
+                        r.refill_u = 0;
+                    }
+                }
+                continue;
+            }
+            if (!more) {
+                break;
+            }
+        }
+        HIPC(hipEventRecord(ev1, stream));
+        HIPC(hipEventSynchronize(ev1));
+        HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        return true;
+    }
+
+    // the protocol event by event with the group's operations
+    float quasistatic_host(const fq::QsArgs& qa, double tol, uint64_t niter_tol, uint64_t max_iter)
+    {
+        auto on = [&](int64_t r) { return !mask || mask[r]; };
+        auto record = [&](int64_t e, int begin) {
+            for (int64_t r = 0; r < R; ++r) {
+                if (!on(r)) {
+                    hrs[(size_t)r].status = fq::ST_IDLE;
+                }
+                else if (hrs[(size_t)r].status == fq::ST_IDLE) {
+                    hrs[(size_t)r].status = fq::ST_CONVERGED;
+                }
+            }
+            push();
+            HIPC(fq::launch_qs_record(views, (int)R, qa, e, begin, stream));
+            pull();
+        };
+        HIPC(hipEventRecord(ev0, stream));
+        for (int64_t e = 0; e < qa.nevents; ++e) {
+            const int64_t k = qa.first + e;
+            record(e, 1);
+            if (k == 0) {
+                for (int64_t r = 0; r < R; ++r) {
+                    if (on(r)) {
+                        hrs[(size_t)r].u_frame = 0.0;
+                    }
+                }
+                forces(fq::FF_FRAME | fq::FF_TOTAL); // u_frame setter, detail.h:1253-1258
+            }
+            else {
+                event_driven_step(qa.eps, (k & 1) == 0, qa.direction);
+            }
+            if ((k & 1) == 0) {
+                if (minim == FQSB_MIN_OVERDAMPED) {
+                    auto conv = nopassing_run(*this, tol, niter_tol, max_iter);
+                    for (int64_t r = 0; r < R; ++r) {
+                        if (on(r) && !conv[(size_t)r]) {
+                            throw Error(hrs[(size_t)r].status == fq::ST_NAN ? "NaN entries found"
+                                                                            : "No convergence found");
+                        }
+                    }
+                }
+                else {
+                    fq::RunArgs ra{};
+                    ra.mode = fq::MODE_MINIMISE;
+                    ra.tol = tol;
+                    ra.tol2 = tol * tol;
+                    ra.niter_tol = (int64_t)niter_tol;
+                    run(ra, max_iter, MIN_BLOCK, false);
+                    for (int64_t r = 0; r < R; ++r) {
+                        const auto st = hrs[(size_t)r].status;
+                        if (on(r) && st != fq::ST_CONVERGED) {
+                            throw Error(st == fq::ST_NAN ? "NaN entries found" : "No convergence found");
+                        }
+                    }
+                }
+            }
+            record(e, 0);
+        }
+        float ms = 0.0f;
+        HIPC(hipEventRecord(ev1, stream));
+        HIPC(hipEventSynchronize(ev1));
+        HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        return ms;
+    }
 };
 
 uint64_t default_stream_state(uint64_t seed, uint64_t& inc)
@@ -1418,6 +1645,59 @@ int64_t checked_p(const Group& g, uint64_t p)
     return (int64_t)p;
 }
 
+// No-passing minimise (detail.h:1694-1753) on the device (Jacobi sweeps) for
+// every system of the group (with g.mask: the masked-in ones). Returns per
+// system true if the StopList converged.
+std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter)
+{
+    for (auto& r : g.hrs) {
+        r.qs_first = r.inc;
+        r.qs_last = r.inc;
+    }
+    g.reset_run(fq::MODE_MINIMISE);
+    g.push();
+    // particles per thread: the fewest for which the launch fits the kernel's
+    // thread bound (1024 up to 8 particles per thread, 512 at 16)
+    int ppt = 1;
+    while (ppt <= 16 && (g.N + ppt - 1) / ppt > (ppt <= 8 ? 1024 : 512)) {
+        ppt *= 2;
+    }
+    if (ppt > 16) {
+        throw Error("system too large for the no-passing kernel (at most 8192 particles)");
+    }
+    const int64_t need = (g.N + ppt - 1) / ppt;
+    const int threads = (int)(((need + 63) / 64) * 64);
+    fq::RunArgs ra{};
+    ra.mode = fq::MODE_MINIMISE;
+    ra.tol = tol;
+    ra.tol2 = tol * tol;
+    ra.niter_tol = (int64_t)niter_tol;
+    uint64_t done = 0;
+    auto running = [&] {
+        for (const auto& r : g.hrs) {
+            if (r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) {
+                return true;
+            }
+        }
+        return false;
+    };
+    while (done < max_iter && running()) {
+        const uint64_t nb = std::min<uint64_t>(MIN_BLOCK, max_iter - done);
+        ra.max_steps = (int64_t)nb;
+        if (!fq::launch_nopassing(ppt, threads, sizeof(double) * 4 * threads, g.views, (int)g.R, ra, g.stream)) {
+            throw Error("no-passing kernel not available");
+        }
+        HIPC(hipGetLastError());
+        done += nb;
+        g.pull();
+    }
+    std::vector<bool> conv((size_t)g.R);
+    for (int64_t r = 0; r < g.R; ++r) {
+        conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
+    }
+    return conv;
+}
+
 } // namespace
 
 struct fqsb_system {
@@ -1696,58 +1976,6 @@ int fqsb_time_steps_until_event(fqsb_system* s, double tol, uint64_t niter_tol, 
     });
 }
 
-// No-passing minimise (detail.h:1694-1753) on the device (Jacobi sweeps) for
-// every system of the group (with g.mask: the masked-in ones). Returns per
-// system true if the StopList converged.
-static std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter)
-{
-    for (auto& r : g.hrs) {
-        r.qs_first = r.inc;
-        r.qs_last = r.inc;
-    }
-    g.reset_run(fq::MODE_MINIMISE);
-    g.push();
-    // particles per thread: the fewest for which the launch fits the kernel's
-    // thread bound (1024 up to 8 particles per thread, 512 at 16)
-    int ppt = 1;
-    while (ppt <= 16 && (g.N + ppt - 1) / ppt > (ppt <= 8 ? 1024 : 512)) {
-        ppt *= 2;
-    }
-    if (ppt > 16) {
-        throw Error("system too large for the no-passing kernel (at most 8192 particles)");
-    }
-    const int64_t need = (g.N + ppt - 1) / ppt;
-    const int threads = (int)(((need + 63) / 64) * 64);
-    fq::RunArgs ra{};
-    ra.mode = fq::MODE_MINIMISE;
-    ra.tol = tol;
-    ra.tol2 = tol * tol;
-    ra.niter_tol = (int64_t)niter_tol;
-    uint64_t done = 0;
-    auto running = [&] {
-        for (const auto& r : g.hrs) {
-            if (r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) {
-                return true;
-            }
-        }
-        return false;
-    };
-    while (done < max_iter && running()) {
-        const uint64_t nb = std::min<uint64_t>(MIN_BLOCK, max_iter - done);
-        ra.max_steps = (int64_t)nb;
-        if (!fq::launch_nopassing(ppt, threads, sizeof(double) * 4 * threads, g.views, (int)g.R, ra, g.stream)) {
-            throw Error("no-passing kernel not available");
-        }
-        HIPC(hipGetLastError());
-        done += nb;
-        g.pull();
-    }
-    std::vector<bool> conv((size_t)g.R);
-    for (int64_t r = 0; r < g.R; ++r) {
-        conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
-    }
-    return conv;
-}
 
 static void minimise_nopassing(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
                                uint64_t* ret)
@@ -1850,6 +2078,15 @@ int fqsb_event_driven_step(fqsb_system* s, double eps, int kick, int direction, 
             throw Error("eventDrivenStep is not available for this system");
         }
         *ret = g.event_driven_step(eps, kick != 0, direction)[0];
+    });
+}
+
+int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, double tol,
+                     uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
+                     int64_t* inc)
+{
+    return guard([&] {
+        s->g->quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, S, u_frame, f_frame, inc);
     });
 }
 
@@ -2143,6 +2380,15 @@ int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direct
         }
         auto r = g.event_driven_step(eps, kick != 0, direction);
         std::copy(r.begin(), r.end(), ret);
+    });
+}
+
+int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+                           double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
+                           double* f_frame, int64_t* inc)
+{
+    return guard([&] {
+        b->g->quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, S, u_frame, f_frame, inc);
     });
 }
 
@@ -2492,6 +2738,64 @@ int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int dire
         });
         for (size_t i = 0; i < r->where.size(); ++i) {
             ret[i] = gret[r->where[i].first][(size_t)r->where[i].second];
+        }
+    });
+}
+
+int fqsb_ragged_quasistatic(fqsb_ragged* r, int64_t first_event, int64_t nevents, double eps, int direction,
+                            double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active, int64_t* S,
+                            double* u_frame, double* f_frame, int64_t* inc)
+{
+    return guard([&] {
+        if (nevents < 0) {
+            throw Error("the first event and the number of events must be non-negative");
+        }
+        auto m = r->masks(active);
+        const size_t G = r->groups.size();
+        std::vector<std::vector<int64_t>> gS(G), ginc(G);
+        std::vector<std::vector<double>> guf(G), gff(G);
+        r->each([&](Group& g, size_t gi) {
+            const size_t n = (size_t)(g.R * nevents);
+            gS[gi].assign(n, 0);
+            ginc[gi].assign(n, 0);
+            guf[gi].assign(n, 0.0);
+            gff[gi].assign(n, 0.0);
+            bool any = false;
+            for (uint8_t x : m[gi]) {
+                any |= x != 0;
+            }
+            if (!any) {
+                return;
+            }
+            g.mask = m[gi].data();
+            try {
+                g.quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, gS[gi].data(),
+                              guf[gi].data(), gff[gi].data(), ginc[gi].data());
+            }
+            catch (...) {
+                g.mask = nullptr;
+                throw;
+            }
+            g.mask = nullptr;
+        });
+        for (size_t i = 0; i < r->where.size(); ++i) {
+            const size_t gi = r->where[i].first;
+            const size_t o = (size_t)r->where[i].second * (size_t)nevents;
+            for (int64_t e = 0; e < nevents; ++e) {
+                const size_t d = i * (size_t)nevents + (size_t)e;
+                if (S) {
+                    S[d] = gS[gi][o + (size_t)e];
+                }
+                if (u_frame) {
+                    u_frame[d] = guf[gi][o + (size_t)e];
+                }
+                if (f_frame) {
+                    f_frame[d] = gff[gi][o + (size_t)e];
+                }
+                if (inc) {
+                    inc[d] = ginc[gi][o + (size_t)e];
+                }
+            }
         }
     });
 }

@@ -56,7 +56,16 @@ enum Status : int32_t {
     ST_NAN = 4,       // NaN in u
     ST_BUDGET = 5,    // step budget of this launch used up
     ST_REFILL = 6,    // the next step needs a landscape refill (k_refill, then relaunch)
-    ST_IDLE = 7       // left out of this run by the caller
+    ST_IDLE = 7,      // left out of this run by the caller
+    ST_DONE = 8,      // quasistatic driver: every event of the run recorded
+    ST_MAXITER = 9    // quasistatic driver: a minimisation used max_iter steps
+};
+
+// Phase of the quasistatic driver (k_quasi) within an event.
+enum QsPhase : int32_t {
+    QS_BEGIN = 0,    // the event's displacement is still to be applied
+    QS_MINIMISE = 1, // minimising after the displacement
+    QS_RECORD = 2    // equilibrium reached: record the event
 };
 
 // Components recomputed by k_forces (fq_service.hpp).
@@ -112,9 +121,15 @@ struct RunState {
     // reductions written by service kernels
     double red0;
     int32_t red_flag;
-    int32_t pad2;
+    int32_t refill_u; // k_refill aligns the particles at u (not at the pending step's slips)
     // uniform displacement of the particles applied by k_forces(FF_ADD_U)
     double du_add;
+    // quasistatic driver: next event of the run (0-based), its phase, and the
+    // sum of (well index - i_ref) over the particles when the event began
+    int64_t qs_event;
+    int32_t qs_phase;
+    int32_t qs_pad;
+    int64_t qs_isum;
 };
 
 // Device view of one system (all pointers are device pointers).
@@ -194,6 +209,26 @@ struct RunArgs {
     double tol, tol2;
     int64_t niter_tol;
     int64_t A_truncate, S_truncate;
+};
+
+// Arguments of the quasistatic driver (k_quasi): events k = first + e,
+// e < nevents, of the protocol
+//   k == 0:       u_frame = 0, then minimise
+//   k odd:        eventDrivenStep(eps, kick = false)
+//   k even > 0:   eventDrivenStep(eps, kick = true), then minimise
+// and, after event e of system r, out_*[r * nevents + e]: S (sum of the well
+// index changes), u_frame, mean frame force, inc.
+struct QsArgs {
+    int64_t first;
+    int64_t nevents;
+    double eps;
+    int32_t direction;
+    int32_t pad;
+    int64_t max_iter; // steps per minimisation
+    int64_t* out_S;
+    double* out_uf;
+    double* out_ff;
+    int64_t* out_inc;
 };
 
 // ---------------------------------------------------------------------------

@@ -38,6 +38,29 @@ FQ_HD double potential_force(const SysView& S, double u, double yl, double yr)
     }
 }
 
+// maxUniformDisplacement of one particle in its current well (yl <= u < yr):
+//   Cuspy      detail.h:177-187  distance to the well's edge in `direction`
+//   SemiSmooth detail.h:283-330  distance to the edge of the parabolic part;
+//              `outside` is set (and +inf returned) when u is outside it
+template <int POT>
+FQ_HD double max_uniform_particle(const SysView& S, double up, double yl, double yr, int direction, int& outside)
+{
+    if constexpr (POT == POT_CUSPY) {
+        return direction > 0 ? yr - up : up - yl;
+    }
+    else {
+        const double mu = S.mu, kappa = S.kappa;
+        double xi = 0.5 * (yl + yr);
+        double u_r = (mu * xi + kappa * yr) / (mu + kappa);
+        double u_l = (mu * xi + kappa * yl) / (mu + kappa);
+        if (up < u_l || !(up <= u_r)) {
+            outside = 1;
+            return __builtin_inf();
+        }
+        return direction > 0 ? u_r - up : up - u_l;
+    }
+}
+
 // Nearest-neighbour chain interactions of particle with slip u and
 // neighbours um (p-1) and up (p+1), periodic.
 //   Laplace1d         detail.h:480-486

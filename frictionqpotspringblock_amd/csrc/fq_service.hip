@@ -32,6 +32,12 @@ hipError_t launch_refill(const SysView* views, int nsys, hipStream_t stream)
     return hipGetLastError();
 }
 
+hipError_t launch_qs_record(const SysView* views, int nsys, const QsArgs& qa, int64_t e, int begin, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_qs_record, dim3(nsys), dim3(SVC_THREADS), 0, stream, views, qa, e, (int32_t)begin);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_iref(const SysView* views, int nsys, int64_t maxN, hipStream_t stream)
 {
     dim3 grid((unsigned)((maxN + SVC_THREADS - 1) / SVC_THREADS), (unsigned)nsys);

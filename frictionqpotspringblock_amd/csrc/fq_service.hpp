@@ -196,23 +196,7 @@ __global__ void __launch_bounds__(SVC_THREADS) k_max_uniform(const SysView* __re
         int32_t j = L.jloc[p];
         double yl = L.y[p * L.nchunk + j], yr = L.y[p * L.nchunk + j + 1];
         align_particle(L, p, up, yl, yr, j);
-        double x;
-        if constexpr (POT == POT_CUSPY) {
-            x = direction > 0 ? yr - up : up - yl;
-        }
-        else {
-            const double mu = S.mu, kappa = S.kappa;
-            double xi = 0.5 * (yl + yr);
-            double u_r = (mu * xi + kappa * yr) / (mu + kappa);
-            double u_l = (mu * xi + kappa * yl) / (mu + kappa);
-            if (up < u_l || !(up <= u_r)) {
-                outside = 1;
-                x = __builtin_inf();
-            }
-            else {
-                x = direction > 0 ? u_r - up : up - u_l;
-            }
-        }
+        const double x = max_uniform_particle<POT>(S, up, yl, yr, direction, outside);
         r = x < r ? x : r;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -253,8 +237,10 @@ __global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double e
 
 // Landscape refill requested by a resident launch (status ST_REFILL): align
 // every particle to its slip after the position update of the pending step,
-// u + dt v + dt^2/2 a (bit-identical to the resident kernel), shifting and
-// regenerating its window like the CPU align; then the run continues.
+// u + dt v + dt^2/2 a (bit-identical to the resident kernel), or, with
+// rs->refill_u set (a uniform displacement of the quasistatic driver), to u
+// itself, shifting and regenerating its window like the CPU align; then the
+// run continues.
 __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restrict__ views)
 {
     const SysView& S = views[blockIdx.x];
@@ -265,8 +251,9 @@ __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restric
     const Landscape& L = S.ls;
     const double dt = S.dt;
     const double c2 = 0.5 * dt * dt;
+    const bool at_u = rs->refill_u != 0;
     for (int64_t p = threadIdx.x; p < S.N; p += blockDim.x) {
-        const double x = S.u[p] + dt * S.v[p] + c2 * S.a[p];
+        const double x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
         const int32_t j = L.jloc[p];
         const double* row = L.y + p * L.nchunk;
         if (!(row[j] <= x && x < row[j + 1])) {
@@ -276,6 +263,7 @@ __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
         rs->status = ST_RUNNING;
+        rs->refill_u = 0;
     }
 }
 
@@ -286,6 +274,63 @@ __global__ void __launch_bounds__(SVC_THREADS) k_set_iref(const SysView* __restr
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < S.N) {
         S.i_ref[p] = S.ls.start[p] + S.ls.jloc[p];
+    }
+}
+
+// Quasistatic protocol driven from the host (systems without a k_quasi
+// variant): begin = 1 stores rs->qs_isum = sum_p (start + jloc - i_ref);
+// begin = 0 records event e, out_*[r * nevents + e] = (sum - qs_isum,
+// u_frame, mean f_frame, inc). Fixed summation order.
+__global__ void __launch_bounds__(SVC_THREADS) k_qs_record(const SysView* __restrict__ views, QsArgs qa, int64_t e,
+                                                           int32_t begin)
+{
+    __shared__ long long s_i[RES_MAX_WAVES];
+    __shared__ double s_d[RES_MAX_WAVES];
+    const SysView& S = views[blockIdx.x];
+    RunState* rs = S.rs;
+    if (rs->status == ST_IDLE) {
+        return;
+    }
+    const Landscape& L = S.ls;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    long long si = 0;
+    double sd = 0.0;
+    for (int64_t p = threadIdx.x; p < S.N; p += blockDim.x) {
+        si += L.start[p] + L.jloc[p] - S.i_ref[p];
+        sd += S.f_frame[p];
+    }
+    si = wave_sum_i(si);
+    sd = wave_sum(sd);
+    if (lane == 0) {
+        s_i[wave] = si;
+        s_d[wave] = sd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long ti = 0;
+        double td = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            ti += s_i[w];
+            td += s_d[w];
+        }
+        if (begin) {
+            rs->qs_isum = ti;
+        }
+        else {
+            const int64_t o = (int64_t)blockIdx.x * qa.nevents + e;
+            if (qa.out_S) {
+                qa.out_S[o] = ti - rs->qs_isum;
+            }
+            if (qa.out_uf) {
+                qa.out_uf[o] = rs->u_frame;
+            }
+            if (qa.out_ff) {
+                qa.out_ff[o] = td / (double)S.N;
+            }
+            if (qa.out_inc) {
+                qa.out_inc[o] = rs->inc;
+            }
+        }
     }
 }
 

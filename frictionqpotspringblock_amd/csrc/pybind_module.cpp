@@ -129,6 +129,22 @@ void athermal(C& cls)
         py::arg("time_activity") = true, py::arg("max_iter_is_error") = true);
     cls.def("eventDrivenStep", &S::eventDrivenStep, "eventDrivenStep", py::arg("eps"), py::arg("kick"),
             py::arg("direction") = 1);
+    cls.def(
+        "quasistatic",
+        [](S& s, double eps, int64_t nevents, int64_t first_event, int direction, double tol, size_t niter_tol,
+           size_t max_iter) {
+            auto q = s.quasistatic(eps, nevents, first_event, direction, tol, niter_tol, max_iter);
+            const std::vector<size_t> sh{q.S.size()};
+            py::dict d;
+            d["S"] = to_array(q.S, sh);
+            d["u_frame"] = to_array(q.u_frame, sh);
+            d["f_frame"] = to_array(q.f_frame, sh);
+            d["inc"] = to_array(q.inc, sh);
+            return d;
+        },
+        "The examples' quasistatic loop (eventDrivenStep + minimise per event) in one call", py::arg("eps"),
+        py::arg("nevents"), py::arg("first_event") = 0, py::arg("direction") = 1, py::arg("tol") = 1e-5,
+        py::arg("niter_tol") = 10, py::arg("max_iter") = size_t(1e9));
     cls.def_property_readonly("quasistaticActivityFirst", &S::quasistaticActivityFirst,
                               "quasistaticActivityFirst");
     cls.def_property_readonly("quasistaticActivityLast", &S::quasistaticActivityLast, "quasistaticActivityLast");

@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from .batch import FIELD, _check, _dp, _ip, _up, capi, make_desc
+from .batch import FIELD, _check, _dp, _ip, _quasistatic, _up, capi, make_desc
 
 _u8p = C.POINTER(C.c_uint8)
 
@@ -39,16 +39,17 @@ _CLASSES = {
 
 class Ragged:
     """A set of systems, each given as ``(class_name, kwargs)`` with the
-    reference's constructor arguments (``Line1d.System_*``; ``Line2d.`` prefix
-    for the 2-d classes), e.g. ``("System_Smooth_Laplace", dict(m=1, ...))``."""
+    reference's constructor arguments (``Line1d.System_*``, with or without the
+    ``Line1d.`` prefix; ``Line2d.`` prefix for the 2-d classes), e.g. ``("System_Smooth_Laplace", dict(m=1, ...))``."""
 
     def __init__(self, systems):
         self._descs = []
         self._shapes = []
         for name, kw in systems:
-            if name not in _CLASSES:
+            key = name[len("Line1d."):] if str(name).startswith("Line1d.") else name
+            if key not in _CLASSES:
                 raise RuntimeError("unknown system class: " + str(name))
-            self._descs.append(make_desc(**_CLASSES[name], **kw))
+            self._descs.append(make_desc(**_CLASSES[key], **kw))
             self._shapes.append([int(s) for s in kw["shape"]])
         n = len(self._descs)
         arr = (type(self._descs[0]) * n)(*self._descs)
@@ -116,6 +117,15 @@ class Ragged:
             )
         )
         return ret
+
+    def quasistatic(self, eps, nevents, first_event=0, direction=1, tol=1e-5, niter_tol=10, max_iter=int(1e9),
+                    active=None):
+        """The examples' quasistatic loop for every active system, each at its
+        own pace (batch.Batch.quasistatic); [systems, nevents] arrays."""
+        return _quasistatic(
+            lambda *a: capi().fqsb_ragged_quasistatic(self._h, *a), self._n, eps, nevents, first_event, direction,
+            tol, niter_tol, max_iter, masked=True, mask=self._mask(active),
+        )
 
     def _split(self, flat):
         return [flat[a:b].reshape(s) for a, b, s in zip(self._offsets[:-1], self._offsets[1:], self._shapes)]

@@ -170,6 +170,26 @@ int fqsb_event_driven_step(fqsb_system* s, double eps, int kick, int direction, 
 int fqsb_trigger(fqsb_system* s, uint64_t p, double eps, int direction);
 int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plastic);
 
+/* The quasistatic protocol of the reference's examples
+ * (examples/Line1d_Cuspy_Laplace.py:39-61, the loop a Python caller writes
+ * around eventDrivenStep and minimise), run as one call: events
+ * k = first_event .. first_event + nevents - 1, where
+ *   k == 0:      u_frame = 0 (detail.h:1253-1258), minimise
+ *   k odd:       eventDrivenStep(eps, false, direction)
+ *   k even > 0:  eventDrivenStep(eps, true, direction), minimise
+ * with minimise(tol, niter_tol, max_iter) raising "No convergence found" at
+ * max_iter as the example's `assert ret == 0`. After event e the host arrays
+ * (each of nevents values, any may be NULL) receive S[e] = the sum over the
+ * particles of the change of chunk.index_at_align during the event, the
+ * frame position u_frame[e], the mean of f_frame[e] and inc[e]. The systems
+ * that fit one workgroup run the whole loop on the device (one workgroup per
+ * system, no host round trip per event); the others, and no-passing systems,
+ * are driven event by event from the host. No counterpart function in the
+ * reference (it replaces the Python loop). */
+int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, double tol,
+                     uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
+                     int64_t* inc);
+
 /* The chunk of yield positions (System::chunk detail.h:1146; the prrng
  * pcg32_tensor_cumsum members used by the reference and its tests:
  * data(), index_at_align(), chunk_index_at_align(), start(), state_at(),
@@ -235,6 +255,10 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
                         uint64_t* ret);
 /* eventDrivenStep on all replicas (each with its own displacement); ret[r]. */
 int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direction, double* ret);
+/* fqsb_quasistatic for every replica, each at its own pace; outputs [R, nevents]. */
+int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+                           double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
+                           double* f_frame, int64_t* inc);
 /* Arrays of all replicas, [R, N]; scalars [R]. */
 int fqsb_batch_get(fqsb_batch* b, int field, double* out);
 int fqsb_batch_set(fqsb_batch* b, int field, const double* in);
@@ -308,6 +332,11 @@ int fqsb_ragged_minimise(fqsb_ragged* r, double tol, uint64_t niter_tol, uint64_
  * maxUniformDisplacement raises, detail.h:414-422). */
 int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int direction, const uint8_t* active,
                                   double* ret);
+/* fqsb_quasistatic for every active system (active: NULL for all);
+ * outputs [systems, nevents], rows of inactive systems zero. */
+int fqsb_ragged_quasistatic(fqsb_ragged* r, int64_t first_event, int64_t nevents, double eps, int direction,
+                            double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active, int64_t* S,
+                            double* u_frame, double* f_frame, int64_t* inc);
 int fqsb_ragged_get(fqsb_ragged* r, int field, double* out);
 int fqsb_ragged_set(fqsb_ragged* r, int field, const double* in);
 int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out);

@@ -457,6 +457,30 @@ public:
     {
         check(fqsb_advance_to_fixed_force(m_h->h, f_frame, allow_plastic));
     }
+
+    // Per-event output of quasistatic(): S, u_frame, mean f_frame, inc.
+    struct Quasistatic {
+        std::vector<int64_t> S;
+        std::vector<double> u_frame;
+        std::vector<double> f_frame;
+        std::vector<int64_t> inc;
+    };
+
+    // The examples' loop around eventDrivenStep and minimise in one call
+    // (fqsb_quasistatic).
+    Quasistatic quasistatic(double eps, int64_t nevents, int64_t first_event = 0, int direction = 1,
+                            double tol = 1e-5, size_t niter_tol = 10, size_t max_iter = size_t(1e9))
+    {
+        Quasistatic q;
+        const size_t n = nevents > 0 ? (size_t)nevents : 0;
+        q.S.resize(n);
+        q.u_frame.resize(n);
+        q.f_frame.resize(n);
+        q.inc.resize(n);
+        check(fqsb_quasistatic(m_h->h, first_event, nevents, eps, direction, tol, niter_tol, max_iter, q.S.data(),
+                               q.u_frame.data(), q.f_frame.data(), q.inc.data()));
+        return q;
+    }
 };
 
 inline fqsb_desc physics(fqsb_desc d, double m, double eta, double mu, double k_frame, double dt)

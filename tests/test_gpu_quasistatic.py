@@ -1,0 +1,204 @@
+"""GPU parity of the quasistatic driver (System.quasistatic, Batch.quasistatic,
+Ragged.quasistatic) against the oracle running the examples' Python loop
+(examples/Line1d_Cuspy_Laplace.py:39-61) event by event.
+
+Avalanche sizes S, increments and frame positions are compared exactly, and
+so is the state after the run (the driver evaluates the same expressions as
+the single operations). The mean frame force is reduced in a fixed order on
+the device and by numpy's pairwise sum on the host: relative tolerance 1e-12.
+"""
+
+import numpy as np
+import pytest
+
+from tests import protocols as P
+from tests.test_gpu_parity import _base, _make, assert_same_state
+
+pytestmark = pytest.mark.gpu
+
+FF_RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def fq():
+    import frictionqpotspringblock_amd as fq
+
+    return fq
+
+
+def assert_same_events(q, u_frame, f_frame, S, inc, what=""):
+    n = len(S)
+    assert np.array_equal(q["S"][-n:], S), (what, q["S"], S)
+    assert np.array_equal(q["inc"][-n:], inc), what
+    assert np.array_equal(q["u_frame"][-n:], u_frame), what
+    np.testing.assert_allclose(q["f_frame"][-n:], f_frame, rtol=FF_RTOL, atol=1e-15, err_msg=what)
+
+
+# (submodule, class, size, extra constructor arguments)
+SYSTEMS = [
+    ("Line1d", "System_Cuspy_Laplace", 1000, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 2000, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 3001, dict(k_interactions=1.0)),
+    ("Line1d", "System_SemiSmooth_Laplace", 700, dict(k_interactions=1.0, kappa=1.0)),
+    ("Line1d", "System_Cuspy_Quartic", 900, dict(a1=1.0, a2=5.0)),
+    ("Line1d", "System_Cuspy_QuarticGradient", 900, dict(k2=1.0, k4=2.0)),
+    ("Line1d", "System_Cuspy_LongRange", 300, dict(k_interactions=1.0, alpha=1.5)),
+    ("Line2d", "System_Cuspy_Laplace", (30, 40), dict(k_interactions=1.0)),
+    ("Line2d", "System_Cuspy_QuarticGradient", (20, 25), dict(k2=1.0, k4=1.0)),
+]
+
+
+@pytest.mark.parametrize("sub,cls,n,extra", SYSTEMS, ids=[f"{s[1]}-{s[0]}-{s[2]}" for s in SYSTEMS])
+def test_driver_matches_protocol(fq, oracle, sub, cls, n, extra):
+    """41 events on the device against the oracle's loop; nchunk 60 makes the
+    driver refill landscapes both during minimisations and after kicks."""
+    a = _make(fq, sub, cls, n, extra)
+    b = _make(oracle, sub, cls, n, extra)
+    q = a.quasistatic(1e-3, 41)
+    ref = P.quasistatic(b, 41)
+    assert_same_events(q, *ref, what=cls)
+    assert np.sum(np.abs(ref[2])) > 0
+    assert_same_state(a, b, "after the driver")
+
+
+@pytest.mark.parametrize("budget", ["7", "1000"])
+def test_driver_relaunches(fq, oracle, monkeypatch, budget):
+    """Launches of a few steps (the run resumes inside minimisations, between
+    events and after refills) and a run split over two calls give the same
+    events and state as the oracle."""
+    monkeypatch.setenv("FQSB_QS_BLOCK", budget)
+    extra = dict(k_interactions=1.0)
+    a = _make(fq, "Line1d", "System_Cuspy_Laplace", 500, extra, nchunk=40)
+    b = _make(oracle, "Line1d", "System_Cuspy_Laplace", 500, extra, nchunk=40)
+    q1 = a.quasistatic(1e-3, 12)
+    q2 = a.quasistatic(1e-3, 17, first_event=12)
+    ref = P.quasistatic(b, 29)
+    q = {k: np.concatenate([q1[k], q2[k]]) for k in q1}
+    assert_same_events(q, *ref)
+    assert_same_state(a, b, "after the split run")
+
+
+def test_direction_left(fq, oracle):
+    extra = dict(k_interactions=1.0)
+    a = _make(fq, "Line1d", "System_Cuspy_Laplace", 600, extra)
+    b = _make(oracle, "Line1d", "System_Cuspy_Laplace", 600, extra)
+    q = a.quasistatic(1e-3, 21, direction=-1)
+    for step in range(21):
+        if step == 0:
+            b.u_frame = 0.0
+        else:
+            b.eventDrivenStep(1e-3, step % 2 == 0, -1)
+        if step % 2 == 0:
+            assert b.minimise() == 0
+    assert q["u_frame"][-1] < 0
+    assert_same_state(a, b, "direction -1")
+
+
+@pytest.mark.parametrize("route", ["host", "tiled", "streaming"])
+def test_host_driven_routes(fq, oracle, monkeypatch, route):
+    """The event-by-event route (systems without a driver kernel: large
+    systems on the tiled / streaming kernels) gives the oracle's events."""
+    if route == "host":
+        monkeypatch.setenv("FQSB_HOST_QUASISTATIC", "1")
+    else:
+        monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
+        if route == "streaming":
+            monkeypatch.setenv("FQSB_NO_TILES", "1")
+    extra = dict(k_interactions=1.0)
+    a = _make(fq, "Line1d", "System_Cuspy_Laplace", 800, extra)
+    b = _make(oracle, "Line1d", "System_Cuspy_Laplace", 800, extra)
+    q = a.quasistatic(1e-3, 15)
+    ref = P.quasistatic(b, 15)
+    assert_same_events(q, *ref, what=route)
+    assert_same_state(a, b, route)
+
+
+def test_nopassing(fq, oracle):
+    par = _base(800, k_interactions=1.0)
+    for k in ("m", "eta", "dt"):
+        par.pop(k)
+    a = fq.Line1d.System_Cuspy_Laplace_Nopassing(**par)
+    b = oracle.Line1d.System_Cuspy_Laplace_Nopassing(**par)
+    q = a.quasistatic(1e-3, 15)
+    ref = P.quasistatic(b, 15)
+    assert_same_events(q, *ref)
+    assert_same_state(a, b, "no-passing")
+
+
+def test_batch_replicas(fq, oracle):
+    """Every replica runs its own event sequence at its own pace; each equals
+    a single oracle system with its seed."""
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R, N = 4, 512
+    kw = dict(
+        potential="Cuspy",
+        interactions="Laplace1d",
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=3,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=60,
+    )
+    batch = Batch(R, seed_stride=1, **kw)
+    q = batch.quasistatic(1e-3, 25)
+    assert q["S"].shape == (R, 25)
+    for r in range(R):
+        o = oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, seed=3 + r))
+        ref = P.quasistatic(o, 25)
+        assert_same_events({k: v[r] for k, v in q.items()}, *ref, what=f"replica {r}")
+        assert np.array_equal(batch.u[r], o.u)
+        assert np.array_equal(batch.index_at_align[r], o.chunk.index_at_align)
+
+
+def test_ragged(fq, oracle):
+    from frictionqpotspringblock_amd.ragged import Ragged
+
+    specs = [
+        ("Line1d", "System_Cuspy_Laplace", 300, dict(k_interactions=1.0)),
+        ("Line1d", "System_SemiSmooth_Laplace", 700, dict(k_interactions=1.0, kappa=1.0)),
+        ("Line1d", "System_Cuspy_Quartic", 1500, dict(a1=1.0, a2=5.0)),
+        ("Line2d", "System_Cuspy_Laplace", (16, 16), dict(k_interactions=1.0)),
+    ]
+    systems = []
+    for i, (sub, cls, n, extra) in enumerate(specs):
+        shape = list(n) if isinstance(n, tuple) else [n]
+        systems.append((f"{sub}.{cls}", _base(int(np.prod(shape)), shape=shape, seed=10 + i, **extra)))
+    rg = Ragged(systems)
+    active = np.array([1, 1, 0, 1], dtype=bool)
+    q = rg.quasistatic(1e-3, 15, active=active)
+    u = rg.u
+    for i, (sub, cls, n, extra) in enumerate(specs):
+        if not active[i]:
+            assert np.all(q["S"][i] == 0) and np.all(q["inc"][i] == 0)
+            continue
+        shape = list(n) if isinstance(n, tuple) else [n]
+        o = getattr(getattr(oracle, sub), cls)(**_base(int(np.prod(shape)), shape=shape, seed=10 + i, **extra))
+        ref = P.quasistatic(o, 15)
+        assert_same_events({k: v[i] for k, v in q.items()}, *ref, what=cls)
+        assert np.array_equal(u[i], o.u)
+
+
+def test_errors(fq):
+    extra = dict(k_interactions=1.0)
+    a = _make(fq, "Line1d", "System_Cuspy_Laplace", 200, extra)
+    with pytest.raises(RuntimeError, match="eps"):
+        a.quasistatic(0.0, 3)
+    with pytest.raises(RuntimeError, match="direction"):
+        a.quasistatic(1e-3, 3, direction=0)
+    with pytest.raises(RuntimeError, match="No convergence found"):
+        a.quasistatic(1e-3, 1, max_iter=5)
+    s = _make(fq, "Line1d", "System_Smooth_Laplace", 200, extra)
+    with pytest.raises(RuntimeError, match="Operation not possible"):
+        s.quasistatic(1e-3, 3)
+    assert s.quasistatic(1e-3, 1)["S"].shape == (1,)
+    # systems with random forcing have no minimise, hence no driver
+    assert not hasattr(fq.Line1d.System_Cuspy_Laplace_RandomForcing, "quasistatic")
+    assert a.quasistatic(1e-3, 0)["S"].shape == (0,)
