@@ -159,11 +159,13 @@ SIGNATURES = {
     "fqsb_batch_event_driven_step": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _dp]),
     "fqsb_batch_quasistatic": (
         C.c_int,
-        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp, _ip],
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
+         _ip],
     ),
     "fqsb_quasistatic": (
         C.c_int,
-        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp, _ip],
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
+         _ip],
     ),
     "fqsb_batch_get": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_batch_set": (C.c_int, [_P, C.c_int, _dp]),
@@ -198,8 +200,8 @@ SIGNATURES = {
     ),
     "fqsb_ragged_quasistatic": (
         C.c_int,
-        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint8),
-         _ip, _dp, _dp, _ip],
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
+         C.POINTER(C.c_uint8), _ip, _dp, _dp, _ip],
     ),
     "fqsb_ragged_get": (C.c_int, [_P, C.c_int, _dp]),
     "fqsb_ragged_set": (C.c_int, [_P, C.c_int, _dp]),
@@ -227,7 +229,7 @@ def capi():
     return _lib
 
 
-def _quasistatic(call, nsys, eps, nevents, first_event, direction, tol, niter_tol, max_iter, masked=False,
+def _quasistatic(call, nsys, eps, nevents, first_event, direction, kicks_only, tol, niter_tol, max_iter, masked=False,
                  mask=None):
     """Shared wrapper of fqsb_*_quasistatic: outputs [nsys, nevents]; `masked`:
     the entry point takes an `active` mask (None: every system)."""
@@ -239,7 +241,8 @@ def _quasistatic(call, nsys, eps, nevents, first_event, direction, tol, niter_to
         f_frame=np.zeros([nsys, n]),
         inc=np.zeros([nsys, n], dtype=np.int64),
     )
-    args = [int(first_event), nevents, float(eps), int(direction), float(tol), int(niter_tol), int(max_iter)]
+    args = [int(first_event), nevents, float(eps), int(direction), int(bool(kicks_only)), float(tol), int(niter_tol),
+            int(max_iter)]
     if masked:
         args.append(None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_uint8)))
     args += [
@@ -465,17 +468,18 @@ class Batch:
         )
         return ret
 
-    def quasistatic(self, eps, nevents, first_event=0, direction=1, tol=1e-5, niter_tol=10, max_iter=int(1e9)):
+    def quasistatic(self, eps, nevents, first_event=0, direction=1, kicks_only=False, tol=1e-5, niter_tol=10,
+                    max_iter=int(1e9)):
         """The examples' quasistatic loop (examples/Line1d_Cuspy_Laplace.py:39-61)
         for every replica, each at its own pace on the device: events
         first_event .. first_event + nevents - 1 (0: u_frame = 0 and minimise;
         odd: eventDrivenStep(eps, False); even: eventDrivenStep(eps, True) and
-        minimise). Returns a dict of [R, nevents] arrays: "S" (sum of the
+        minimise; kicks_only: every event > 0 a kick and minimise). Returns a dict of [R, nevents] arrays: "S" (sum of the
         changes of chunk.index_at_align), "u_frame", "f_frame" (mean frame
         force) and "inc" after each event."""
         return _quasistatic(
             lambda *a: capi().fqsb_batch_quasistatic(self._h, *a), self._R, eps, nevents, first_event, direction,
-            tol, niter_tol, max_iter,
+            kicks_only, tol, niter_tol, max_iter,
         )
 
     def _get(self, name):

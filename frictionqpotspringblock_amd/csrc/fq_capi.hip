@@ -1063,14 +1063,16 @@ struct Group {
     //   k == 0:      u_frame = 0, minimise
     //   k odd:       eventDrivenStep(eps, false, direction)
     //   k even > 0:  eventDrivenStep(eps, true, direction), minimise
+    // or, with kicks set, eventDrivenStep(eps, true, direction) and minimise
+    // for every k > 0.
     // recording after event e, at [r * nevents + e] of the host arrays (any
     // may be null): S (sum of well-index changes), u_frame, mean f_frame, inc.
     // Systems that fit a resident workgroup run the whole protocol on the
     // device (k_quasi, one launch per step budget, refill or end); the others
     // are driven event by event from here. Returns the time around the device
     // work (ms).
-    float quasistatic(int64_t first, int64_t nevents, double eps, int direction, double tol, uint64_t niter_tol,
-                      uint64_t max_iter, int64_t* S, double* uf, double* ff, int64_t* inc_out)
+    float quasistatic(int64_t first, int64_t nevents, double eps, int direction, int kicks, double tol,
+                      uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* uf, double* ff, int64_t* inc_out)
     {
         if (first < 0 || nevents < 0) {
             throw Error("the first event and the number of events must be non-negative");
@@ -1087,7 +1089,9 @@ struct Group {
         if (minim == FQSB_MIN_NONE) {
             throw Error("Minimisation not implementated");
         }
-        if (pot == FQSB_POT_SMOOTH && nevents > (first == 0 ? 1 : 0)) {
+        // an elastic event needs maxUniformDisplacement, which Smooth raises
+        const bool elastic = !kicks && nevents > 0 && (nevents > 1 || (first & 1));
+        if (pot == FQSB_POT_SMOOTH && elastic) {
             throw Error("Operation not possible.");
         }
         if (ext) {
@@ -1106,6 +1110,7 @@ struct Group {
         qa.nevents = nevents;
         qa.eps = eps;
         qa.direction = direction;
+        qa.kicks = kicks ? 1 : 0;
         qa.max_iter = (int64_t)std::min<uint64_t>(max_iter, (uint64_t)std::numeric_limits<int64_t>::max());
         qa.out_S = dS;
         qa.out_uf = duf;
@@ -1245,9 +1250,9 @@ struct Group {
                 forces(fq::FF_FRAME | fq::FF_TOTAL); // u_frame setter, detail.h:1253-1258
             }
             else {
-                event_driven_step(qa.eps, (k & 1) == 0, qa.direction);
+                event_driven_step(qa.eps, qa.kicks || (k & 1) == 0, qa.direction);
             }
-            if ((k & 1) == 0) {
+            if (qa.kicks || (k & 1) == 0) {
                 if (minim == FQSB_MIN_OVERDAMPED) {
                     auto conv = nopassing_run(*this, tol, niter_tol, max_iter);
                     for (int64_t r = 0; r < R; ++r) {
@@ -2081,12 +2086,13 @@ int fqsb_event_driven_step(fqsb_system* s, double eps, int kick, int direction, 
     });
 }
 
-int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, double tol,
-                     uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
+int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
+                     double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
                      int64_t* inc)
 {
     return guard([&] {
-        s->g->quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, S, u_frame, f_frame, inc);
+        s->g->quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame,
+                          inc);
     });
 }
 
@@ -2383,12 +2389,13 @@ int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direct
     });
 }
 
-int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
                            double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
                            double* f_frame, int64_t* inc)
 {
     return guard([&] {
-        b->g->quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, S, u_frame, f_frame, inc);
+        b->g->quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame,
+                          inc);
     });
 }
 
@@ -2743,8 +2750,8 @@ int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int dire
 }
 
 int fqsb_ragged_quasistatic(fqsb_ragged* r, int64_t first_event, int64_t nevents, double eps, int direction,
-                            double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active, int64_t* S,
-                            double* u_frame, double* f_frame, int64_t* inc)
+                            int kicks, double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active,
+                            int64_t* S, double* u_frame, double* f_frame, int64_t* inc)
 {
     return guard([&] {
         if (nevents < 0) {
@@ -2769,7 +2776,7 @@ int fqsb_ragged_quasistatic(fqsb_ragged* r, int64_t first_event, int64_t nevents
             }
             g.mask = m[gi].data();
             try {
-                g.quasistatic(first_event, nevents, eps, direction, tol, niter_tol, max_iter, gS[gi].data(),
+                g.quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, gS[gi].data(),
                               guf[gi].data(), gff[gi].data(), ginc[gi].data());
             }
             catch (...) {

@@ -216,6 +216,7 @@ struct RunArgs {
 //   k == 0:       u_frame = 0, then minimise
 //   k odd:        eventDrivenStep(eps, kick = false)
 //   k even > 0:   eventDrivenStep(eps, kick = true), then minimise
+// (kicks != 0: every k > 0 a kick followed by minimise)
 // and, after event e of system r, out_*[r * nevents + e]: S (sum of the well
 // index changes), u_frame, mean frame force, inc.
 struct QsArgs {
@@ -223,7 +224,7 @@ struct QsArgs {
     int64_t nevents;
     double eps;
     int32_t direction;
-    int32_t pad;
+    int32_t kicks;
     int64_t max_iter; // steps per minimisation
     int64_t* out_S;
     double* out_uf;

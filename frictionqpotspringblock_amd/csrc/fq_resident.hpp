@@ -520,7 +520,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 uf = 0.0;
                 phase = QS_MINIMISE;
             }
-            else if (k & 1) { // elastic: eventDrivenStep(eps, false)
+            else if ((k & 1) && !qa.kicks) { // elastic: eventDrivenStep(eps, false)
                 double m = __builtin_inf();
                 int outside = 0;
 #pragma unroll

@@ -131,9 +131,9 @@ void athermal(C& cls)
             py::arg("direction") = 1);
     cls.def(
         "quasistatic",
-        [](S& s, double eps, int64_t nevents, int64_t first_event, int direction, double tol, size_t niter_tol,
-           size_t max_iter) {
-            auto q = s.quasistatic(eps, nevents, first_event, direction, tol, niter_tol, max_iter);
+        [](S& s, double eps, int64_t nevents, int64_t first_event, int direction, bool kicks_only, double tol,
+           size_t niter_tol, size_t max_iter) {
+            auto q = s.quasistatic(eps, nevents, first_event, direction, kicks_only, tol, niter_tol, max_iter);
             const std::vector<size_t> sh{q.S.size()};
             py::dict d;
             d["S"] = to_array(q.S, sh);
@@ -143,7 +143,8 @@ void athermal(C& cls)
             return d;
         },
         "The examples' quasistatic loop (eventDrivenStep + minimise per event) in one call", py::arg("eps"),
-        py::arg("nevents"), py::arg("first_event") = 0, py::arg("direction") = 1, py::arg("tol") = 1e-5,
+        py::arg("nevents"), py::arg("first_event") = 0, py::arg("direction") = 1, py::arg("kicks_only") = false,
+        py::arg("tol") = 1e-5,
         py::arg("niter_tol") = 10, py::arg("max_iter") = size_t(1e9));
     cls.def_property_readonly("quasistaticActivityFirst", &S::quasistaticActivityFirst,
                               "quasistaticActivityFirst");

@@ -118,13 +118,13 @@ class Ragged:
         )
         return ret
 
-    def quasistatic(self, eps, nevents, first_event=0, direction=1, tol=1e-5, niter_tol=10, max_iter=int(1e9),
-                    active=None):
+    def quasistatic(self, eps, nevents, first_event=0, direction=1, kicks_only=False, tol=1e-5, niter_tol=10,
+                    max_iter=int(1e9), active=None):
         """The examples' quasistatic loop for every active system, each at its
         own pace (batch.Batch.quasistatic); [systems, nevents] arrays."""
         return _quasistatic(
             lambda *a: capi().fqsb_ragged_quasistatic(self._h, *a), self._n, eps, nevents, first_event, direction,
-            tol, niter_tol, max_iter, masked=True, mask=self._mask(active),
+            kicks_only, tol, niter_tol, max_iter, masked=True, mask=self._mask(active),
         )
 
     def _split(self, flat):

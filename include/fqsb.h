@@ -177,7 +177,8 @@ int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plasti
  *   k == 0:      u_frame = 0 (detail.h:1253-1258), minimise
  *   k odd:       eventDrivenStep(eps, false, direction)
  *   k even > 0:  eventDrivenStep(eps, true, direction), minimise
- * with minimise(tol, niter_tol, max_iter) raising "No convergence found" at
+ * (kicks != 0: every k > 0 is eventDrivenStep(eps, true, direction) and
+ * minimise, the form available to the Smooth potential) with minimise(tol, niter_tol, max_iter) raising "No convergence found" at
  * max_iter as the example's `assert ret == 0`. After event e the host arrays
  * (each of nevents values, any may be NULL) receive S[e] = the sum over the
  * particles of the change of chunk.index_at_align during the event, the
@@ -186,8 +187,8 @@ int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plasti
  * system, no host round trip per event); the others, and no-passing systems,
  * are driven event by event from the host. No counterpart function in the
  * reference (it replaces the Python loop). */
-int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, double tol,
-                     uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
+int fqsb_quasistatic(fqsb_system* s, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
+                     double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame, double* f_frame,
                      int64_t* inc);
 
 /* The chunk of yield positions (System::chunk detail.h:1146; the prrng
@@ -256,7 +257,7 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
 /* eventDrivenStep on all replicas (each with its own displacement); ret[r]. */
 int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direction, double* ret);
 /* fqsb_quasistatic for every replica, each at its own pace; outputs [R, nevents]. */
-int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
                            double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
                            double* f_frame, int64_t* inc);
 /* Arrays of all replicas, [R, N]; scalars [R]. */
@@ -335,8 +336,8 @@ int fqsb_ragged_event_driven_step(fqsb_ragged* r, double eps, int kick, int dire
 /* fqsb_quasistatic for every active system (active: NULL for all);
  * outputs [systems, nevents], rows of inactive systems zero. */
 int fqsb_ragged_quasistatic(fqsb_ragged* r, int64_t first_event, int64_t nevents, double eps, int direction,
-                            double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active, int64_t* S,
-                            double* u_frame, double* f_frame, int64_t* inc);
+                            int kicks, double tol, uint64_t niter_tol, uint64_t max_iter, const uint8_t* active,
+                            int64_t* S, double* u_frame, double* f_frame, int64_t* inc);
 int fqsb_ragged_get(fqsb_ragged* r, int field, double* out);
 int fqsb_ragged_set(fqsb_ragged* r, int field, const double* in);
 int fqsb_ragged_get_u_frame(fqsb_ragged* r, double* out);

@@ -469,7 +469,8 @@ public:
     // The examples' loop around eventDrivenStep and minimise in one call
     // (fqsb_quasistatic).
     Quasistatic quasistatic(double eps, int64_t nevents, int64_t first_event = 0, int direction = 1,
-                            double tol = 1e-5, size_t niter_tol = 10, size_t max_iter = size_t(1e9))
+                            bool kicks_only = false, double tol = 1e-5, size_t niter_tol = 10,
+                            size_t max_iter = size_t(1e9))
     {
         Quasistatic q;
         const size_t n = nevents > 0 ? (size_t)nevents : 0;
@@ -477,8 +478,8 @@ public:
         q.u_frame.resize(n);
         q.f_frame.resize(n);
         q.inc.resize(n);
-        check(fqsb_quasistatic(m_h->h, first_event, nevents, eps, direction, tol, niter_tol, max_iter, q.S.data(),
-                               q.u_frame.data(), q.f_frame.data(), q.inc.data()));
+        check(fqsb_quasistatic(m_h->h, first_event, nevents, eps, direction, kicks_only, tol, niter_tol, max_iter,
+                               q.S.data(), q.u_frame.data(), q.f_frame.data(), q.inc.data()));
         return q;
     }
 };

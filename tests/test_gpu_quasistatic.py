@@ -94,6 +94,38 @@ def test_direction_left(fq, oracle):
     assert_same_state(a, b, "direction -1")
 
 
+def kicks_protocol(system, nstep, eps=1e-3):
+    """The loop with a kick before every minimisation (the form available to
+    the Smooth potential, whose maxUniformDisplacement raises)."""
+    out = [np.empty(nstep), np.empty(nstep), np.empty(nstep, dtype=np.int64), np.empty(nstep, dtype=np.int64)]
+    for step in range(nstep):
+        i_n = np.copy(system.chunk.index_at_align)
+        if step == 0:
+            system.u_frame = 0.0
+        else:
+            system.eventDrivenStep(eps, True)
+        assert system.minimise() == 0
+        out[0][step] = system.u_frame
+        out[1][step] = np.mean(system.f_frame)
+        out[2][step] = np.sum(system.chunk.index_at_align - i_n)
+        out[3][step] = system.inc
+    return out
+
+
+@pytest.mark.parametrize("cls,extra", [("System_Smooth_Laplace", {}), ("System_Cuspy_Laplace", {})])
+@pytest.mark.parametrize("host", [False, True])
+def test_kicks_only(fq, oracle, monkeypatch, cls, extra, host):
+    if host:
+        monkeypatch.setenv("FQSB_HOST_QUASISTATIC", "1")
+    extra = dict(k_interactions=1.0, **extra)
+    a = _make(fq, "Line1d", cls, 700, extra)
+    b = _make(oracle, "Line1d", cls, 700, extra)
+    q = a.quasistatic(1e-3, 12, kicks_only=True)
+    ref = kicks_protocol(b, 12)
+    assert_same_events(q, *ref, what=cls)
+    assert_same_state(a, b, "kicks only")
+
+
 @pytest.mark.parametrize("route", ["host", "tiled", "streaming"])
 def test_host_driven_routes(fq, oracle, monkeypatch, route):
     """The event-by-event route (systems without a driver kernel: large
