@@ -24,6 +24,15 @@ of events with the no-passing minimiser on the Cuspy-Laplace chains (up to
 8192 particles); value = sum over systems of N x (time steps or sweeps) per
 second.
 
+--workload quasistatic: R = 1024 replicas of the reference's CPU configuration
+(Line1d Cuspy-Laplace, N = 1024, parameters as above with k_frame = 1/1024,
+replica r with seed r), each running the examples' quasistatic protocol
+(examples/Line1d_Cuspy_Laplace.py:39-61) at its own pace on the device; one
+bench step = one event pair (elastic step to the next yield, then kick and
+minimise) on every replica, avalanche sizes recorded; the default --steps 100
+is the configuration's 100 events. value = sum over replicas of N x time steps
+per second.
+
 --workload longrange: R = 256 replicas of N = 8192 particles with LongRange1d
 interactions (alpha = 1.5, k = 1) and random normal forcing (mean 0, stddev
 0.1, dinc = dinc_init = 10), otherwise as above; timeSteps with the
@@ -67,6 +76,11 @@ LR_N = 8192
 LR_BLOCK = 100
 
 RG_EPS = 1e-3
+QS_R = 1024
+QS_N = 1024
+QS_EPS = 1e-3
+FLOP_PER_PARTICLE_STEP_MIN = 44.0  # the Verlet step plus the residual norms of the StopList
+QS_SHAPES = [(1, 1024), (2, 512), (4, 256), (8, 128)]  # (particles per thread, threads) for N = 1024
 
 AD_R = 64
 AD_N = 4096
@@ -461,6 +475,141 @@ def rg_cpu_baseline(target_s=10.0):
     }
 
 
+def qs_params(seed=0):
+    return dict(PARAMS, k_frame=1.0 / QS_N, shape=[QS_N], seed=seed)
+
+
+def qs_cpu_baseline(target_s=10.0):
+    """The oracle running the examples' protocol on replicas of the quasistatic
+    workload, one per thread, for about `target_s`; sum N x time steps."""
+    from oracle import pyoracle
+
+    pyoracle.lib()
+    cores = min(16, os.cpu_count() or 1)
+    systems = [pyoracle.Line1d.System_Cuspy_Laplace(**qs_params(r)) for r in range(cores)]
+    for s in systems:
+        s.u_frame = 0.0
+        s.minimise()
+
+    def pair(s):
+        i0 = s.inc
+        s.eventDrivenStep(QS_EPS, False)
+        s.eventDrivenStep(QS_EPS, True)
+        s.minimise()
+        return QS_N * (s.inc - i0)
+
+    work, pairs, t0 = 0, 0, time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        while time.perf_counter() - t0 < target_s:
+            work += sum(ex.map(pair, systems))
+            pairs += 1
+    dt = time.perf_counter() - t0
+    return {
+        "value": work / dt,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{cores} replicas (seeds 0..{cores - 1}), one per thread, {pairs} event pairs each of the "
+        f"same protocol ({dt:.1f} s)",
+    }
+
+
+def qs_plain_path(pairs=5):
+    """One system through the reference's API, the examples' Python loop."""
+    import torch
+
+    import frictionqpotspringblock_amd as fq
+
+    s = fq.Line1d.System_Cuspy_Laplace(**qs_params(0))
+    s.u_frame = 0.0
+    s.minimise()
+    torch.cuda.synchronize()
+    i0, t0 = s.inc, time.perf_counter()
+    for _ in range(pairs):
+        s.eventDrivenStep(QS_EPS, False)
+        s.eventDrivenStep(QS_EPS, True)
+        s.minimise()
+    torch.cuda.synchronize()
+    return QS_N * (s.inc - i0) / (time.perf_counter() - t0)
+
+
+def run_quasistatic(args):
+    import torch
+
+    from frictionqpotspringblock_amd.batch import Batch
+
+    torch.cuda.init()
+    batch = Batch(QS_R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **qs_params(0))
+    batch.quasistatic(QS_EPS, 1)  # event 0: the initial quench
+    event = 1
+    tune = {}
+    if not args.no_autotune:
+        for ppt, threads in QS_SHAPES:  # one event pair per launch shape
+            batch.set_launch(ppt, threads)
+            batch.quasistatic(QS_EPS, 2, first_event=event)
+            event += 2
+            tune[f"{ppt}x{threads}"] = batch.quasistatic_stats[0]
+        best = min(tune, key=tune.get)
+        batch.set_launch(*[int(x) for x in best.split("x")])
+    if args.warmup:
+        batch.quasistatic(QS_EPS, 2 * args.warmup, first_event=event)
+        event += 2 * args.warmup
+    inc0 = batch.inc.copy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q = batch.quasistatic(QS_EPS, 2 * args.steps, first_event=event)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, launches = batch.quasistatic_stats
+    steps = batch.inc - inc0
+    work = float(QS_N * steps.sum())
+    flop = FLOP_PER_PARTICLE_STEP_MIN * work
+    achieved = flop / (ms * 1e-3) / 1e12
+    ppt, threads = batch.launch
+    plain = None if args.no_plain else qs_plain_path()
+    cpu = None if args.no_cpu else qs_cpu_baseline()
+    return {
+        "metric": METRIC,
+        "value": work / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Weibull yield landscapes from pcg32, replica r with seed r)",
+        "config": {
+            "workload": f"quasistatic: {QS_R} Line1d Cuspy-Laplace replicas x N={QS_N}, {args.steps} event pairs "
+            "(elastic step, kick + minimise) each of the examples' protocol, device-side driver",
+            "replicas": QS_R,
+            "N": QS_N,
+            "event_pairs": args.steps,
+            "time_steps_per_replica_mean": float(steps.mean()),
+            "time_steps_per_replica_max": int(steps.max()),
+            "avalanche_size_mean": float(np.abs(q["S"][:, 1::2]).mean()),
+            "launch": f"{ppt}x{threads}",
+            "launch_tuning_ms": tune,
+            "driver_launches": launches,
+            "plain_one_system_python_loop": plain,
+        },
+        "roofline": {
+            "kernel": f"k_quasi<Cuspy, Laplace1d, {ppt} x {threads}>",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": 78.6,
+            "unit": "TFLOP/s",
+            "frac": achieved / 78.6,
+            "traffic": None,
+            "flop_per_particle_step": FLOP_PER_PARTICLE_STEP_MIN,
+            "kernel_ms_total": ms,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
 def run_ragged(args):
     import torch
 
@@ -599,12 +748,14 @@ def run_flow(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["flow", "longrange", "adjoint", "ragged"], default="flow")
+    ap.add_argument(
+        "--workload", choices=["flow", "longrange", "adjoint", "ragged", "quasistatic"], default="flow"
+    )
     ap.add_argument(
         "--steps", type=int, default=None,
         help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
         "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5; "
-        "ragged: quasistatic events, default 20)",
+        "ragged: quasistatic events, default 20; quasistatic: event pairs, default 100)",
     )
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -633,10 +784,14 @@ def main():
         )
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 20}[args.workload]
-    run = {"flow": run_flow, "longrange": run_longrange, "adjoint": run_adjoint, "ragged": run_ragged}[
-        args.workload
-    ]
+        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 20, "quasistatic": 100}[args.workload]
+    run = {
+        "flow": run_flow,
+        "longrange": run_longrange,
+        "adjoint": run_adjoint,
+        "ragged": run_ragged,
+        "quasistatic": run_quasistatic,
+    }[args.workload]
     line = run(args)
     print(json.dumps(line), flush=True)
 

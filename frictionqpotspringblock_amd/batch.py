@@ -162,6 +162,7 @@ SIGNATURES = {
         [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
          _ip],
     ),
+    "fqsb_batch_quasistatic_stats": (C.c_int, [_P, _fp, _ip]),
     "fqsb_quasistatic": (
         C.c_int,
         [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
@@ -481,6 +482,15 @@ class Batch:
             lambda *a: capi().fqsb_batch_quasistatic(self._h, *a), self._R, eps, nevents, first_event, direction,
             kicks_only, tol, niter_tol, max_iter,
         )
+
+    @property
+    def quasistatic_stats(self):
+        """(ms, launches) of the last quasistatic(): time between HIP events
+        around its device work and the number of driver launches."""
+        ms = C.c_float()
+        n = C.c_int64()
+        _check(capi().fqsb_batch_quasistatic_stats(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def _get(self, name):
         out = np.empty(self._R * self._N)

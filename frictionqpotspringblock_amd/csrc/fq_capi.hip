@@ -165,6 +165,10 @@ struct Group {
     int lr_nsplit = 1;
     hipEvent_t evg0 = nullptr, evg1 = nullptr; // around one product launch per block (timed runs)
     float lr_product_ms = 0.0f;                // average product duration of the last timed run
+    // last quasistatic run: time around its device work and number of
+    // driver launches (0: driven from the host)
+    float qs_ms = 0.0f;
+    int64_t qs_launches = 0;
     double *lr_un = nullptr, *lr_part = nullptr, *lr_yl = nullptr, *lr_yr = nullptr;
 
     Group() = default;
@@ -1117,6 +1121,7 @@ struct Group {
         qa.out_ff = dff;
         qa.out_inc = dinc;
         float ms = 0.0f;
+        qs_launches = 0;
         try {
             set_iref_current();
             const bool device = !streaming && minim == FQSB_MIN_INERTIAL &&
@@ -1146,6 +1151,7 @@ struct Group {
         dfree(duf);
         dfree(dff);
         dfree(dinc);
+        qs_ms = ms;
         return ms;
     }
 
@@ -1164,7 +1170,6 @@ struct Group {
             r.qs_event = 0;
             r.qs_phase = fq::QS_BEGIN;
             r.qs_isum = 0;
-            r.refill_u = 0;
         }
         fq::RunArgs ra{};
         ra.mode = fq::MODE_MINIMISE;
@@ -1187,6 +1192,7 @@ struct Group {
                 throw Error("quasistatic kernel not available");
             }
             HIPC(hipGetLastError());
+            ++qs_launches;
             pull();
             bool refill = false, more = false;
             for (const auto& r : hrs) {
@@ -1205,7 +1211,6 @@ struct Group {
                 for (auto& r : hrs) {
                     if (r.status == fq::ST_REFILL) {
                         r.status = fq::ST_RUNNING;
-                        r.refill_u = 0;
                     }
                 }
                 continue;
@@ -2396,6 +2401,14 @@ int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, 
     return guard([&] {
         b->g->quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame,
                           inc);
+    });
+}
+
+int fqsb_batch_quasistatic_stats(const fqsb_batch* b, float* ms, int64_t* launches)
+{
+    return guard([&] {
+        *ms = b->g->qs_ms;
+        *launches = b->g->qs_launches;
     });
 }
 

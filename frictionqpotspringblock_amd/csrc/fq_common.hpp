@@ -121,7 +121,7 @@ struct RunState {
     // reductions written by service kernels
     double red0;
     int32_t red_flag;
-    int32_t refill_u; // k_refill aligns the particles at u (not at the pending step's slips)
+    int32_t pad2;
     // uniform displacement of the particles applied by k_forces(FF_ADD_U)
     double du_add;
     // quasistatic driver: next event of the run (0-based), its phase, and the

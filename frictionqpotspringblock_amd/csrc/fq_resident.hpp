@@ -176,13 +176,38 @@ FQ_HD size_t resident_lds_bytes(int64_t N, int threads)
     }
 }
 
+// Landscape refill inside a resident launch, by the thread that owns
+// particles base .. base + cnt - 1 (only it reads their rows): align each to
+// the slip of the pending step, u + dt v + dt^2/2 a, or to u itself (at_u, a
+// uniform displacement of the quasistatic driver), from the state in memory,
+// exactly as k_refill does. Called with the state written back, so that no
+// registers are live across the call.
+__device__ __noinline__ void refill_own(const SysView* __restrict__ views, int64_t base, int cnt, int at_u)
+{
+    const SysView& S = views[blockIdx.x];
+    const Landscape& L = S.ls;
+    const double dt = S.dt;
+    const double c2 = 0.5 * dt * dt;
+    for (int i = 0; i < cnt; ++i) {
+        const int64_t p = base + i;
+        const double x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
+        const int32_t j = L.jloc[p];
+        const double* row = L.y + p * L.nchunk;
+        if (!(row[j] <= x && x < row[j + 1])) {
+            chunk_align_slow(L, p, x, j);
+        }
+    }
+}
+
 // The body of k_resident (QS = false) and of the quasistatic driver k_quasi
 // (QS = true). The driver wraps the minimise loop in the event protocol of
 // QsArgs: when the StopList is met it quenches, records the event, applies
 // the next events' displacements (eventDrivenStep, detail.h:1933-1960, with
 // advanceUniformly detail.h:2027-2050) and restarts the minimisation at the
-// same iteration, all inside the workgroup; it leaves the launch only for a
-// refill, the step budget, a NaN, max_iter or the end of the run.
+// same iteration, all inside the workgroup; it leaves the launch only for the
+// step budget, a NaN, max_iter or the end of the run.
+// A step (or displacement) that needs a landscape refill writes the state
+// back, refills (refill_own) and re-enters with the state reloaded.
 template <int POT, int INT, int EXT, int PPT, int MAXT, bool QS>
 __device__ __forceinline__ void resident_body(const SysView* __restrict__ views, const RunArgs& ra, const QsArgs& qa)
 {
@@ -192,10 +217,6 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
     if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
         return;
     }
-    // steps this launch may take: the launch budget, capped by the run's target
-    // (the quasistatic driver: the launch budget; max_iter is per minimisation)
-    const int64_t room = QS ? ra.max_steps : rs->target - rs->steps;
-    const int limit = (int)(room < ra.max_steps ? (room > 0 ? room : 0) : ra.max_steps);
 
     __shared__ double s_red_d[2][RES_MAX_WAVES][2];
     __shared__ long long s_red_i[2][RES_MAX_WAVES][2];
@@ -230,79 +251,6 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
     double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], fth[PPT];
     int32_t jl[PPT], dl[PPT];
 
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        u[i] = v[i] = a[i] = yl[i] = yr[i] = fth[i] = 0.0;
-        jl[i] = 0;
-        dl[i] = 0;
-        if (i < cnt) {
-            const int64_t p = base + i;
-            u[i] = S.u[p];
-            v[i] = S.v[p];
-            a[i] = S.a[p];
-            jl[i] = L.jloc[p];
-            yl[i] = Y[p * nch + jl[i]];
-            yr[i] = Y[p * nch + jl[i] + 1];
-            if constexpr (EXT) {
-                fth[i] = S.f_th[p];
-            }
-            if (track) {
-                dl[i] = (int32_t)(L.start[p] + jl[i] - S.i_ref[p]);
-            }
-        }
-    }
-
-    const double dt = S.dt;
-    const double c2 = 0.5 * dt * dt;
-    const double hdt = 0.5 * dt;
-    const double meta = -S.eta;
-    const double inv_m = S.inv_m;
-    const double kf = S.k_frame;
-    const double dv = ra.v_frame * dt;
-
-    // Publish the slips that other threads read: the current ones (`next`
-    // false) or those after the position update of the coming step, computed
-    // without touching the registers.
-    auto publish = [&](int b, bool next) {
-        if constexpr (CHAIN) {
-            if (cnt > 0) {
-#pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    if (i == 0 || i == cnt - 1) {
-                        const double x = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
-                        if (i == 0) {
-                            h_first[b * T + t] = x;
-                        }
-                        if (i == cnt - 1) {
-                            h_last[b * T + t] = x;
-                        }
-                    }
-                }
-            }
-        }
-        else {
-#pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                if (i < cnt) {
-                    mir[(int64_t)b * N + base + i] = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
-                }
-            }
-        }
-    };
-
-    if constexpr (INT == INT_LONGRANGE1D) {
-        for (int64_t q = t; q < N; q += T) {
-            pref[q] = S.pref[q];
-        }
-    }
-    publish(0, false);
-    if (t == 0) {
-        s_nan[0] = 0;
-        s_nan[1] = 0;
-        s_refill[0] = 0;
-        s_refill[1] = 0;
-    }
-
     int64_t inc = rs->inc;
     double uf = rs->u_frame;
     int32_t below = rs->below, below2 = rs->below2, dec = rs->dec;
@@ -315,8 +263,8 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
     int64_t min_next = rs->min_next;
 
     // quasistatic driver: event, phase, sum of dl when the event began, steps
-    // of the current minimisation before this launch, the iteration at which
-    // it (re)started in this launch, and whether k_refill must align at u
+    // of the current minimisation before this entry, the iteration at which
+    // it (re)started in this entry, and whether a refill aligns at u
     int64_t ev = 0, isum = 0, ev_done0 = 0;
     int32_t phase = QS_MINIMISE, refill_u = 0;
     int it0 = 1;
@@ -326,390 +274,191 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         isum = rs->qs_isum;
         ev_done0 = rs->steps;
     }
+    int64_t launch_done = 0; // steps of the earlier entries of this launch
+    int32_t status;
+    int it;
 
-    if constexpr (EXT) {
-        long long mn = 0x7fffffffffffffffLL;
+    for (;;) { // entries: the first, then one after each in-kernel refill
+        // steps this entry may take: what is left of the launch budget, capped by
+        // the run's target (the quasistatic driver: max_iter is per minimisation)
+        const int64_t budget = ra.max_steps - launch_done;
+        const int64_t room = QS ? budget : rs->target - rs->steps - launch_done;
+        const int limit = (int)(room < budget ? (room > 0 ? room : 0) : budget);
+
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
+            u[i] = v[i] = a[i] = yl[i] = yr[i] = fth[i] = 0.0;
+            jl[i] = 0;
+            dl[i] = 0;
             if (i < cnt) {
-                long long nx = S.next[base + i];
-                mn = nx < mn ? nx : mn;
+                const int64_t p = base + i;
+                u[i] = S.u[p];
+                v[i] = S.v[p];
+                a[i] = S.a[p];
+                jl[i] = L.jloc[p];
+                yl[i] = Y[p * nch + jl[i]];
+                yr[i] = Y[p * nch + jl[i] + 1];
+                if constexpr (EXT) {
+                    fth[i] = S.f_th[p];
+                }
+                if (track) {
+                    dl[i] = (int32_t)(L.start[p] + jl[i] - S.i_ref[p]);
+                }
             }
         }
-        min_next = uniform_ll(block_min_i(mn, s_min_w));
-    }
-    __syncthreads();
 
-    auto left_of = [&](int b) -> double { return h_last[b * T + (t == 0 ? Tact - 1 : t - 1)]; };
-    auto right_of = [&](int b) -> double { return h_first[b * T + (t == Tact - 1 ? 0 : t + 1)]; };
+        const double dt = S.dt;
+        const double c2 = 0.5 * dt * dt;
+        const double hdt = 0.5 * dt;
+        const double meta = -S.eta;
+        const double inv_m = S.inv_m;
+        const double kf = S.k_frame;
+        const double dv = ra.v_frame * dt;
 
-    // interaction force of local particle i, neighbours from exchange buffer b
-    auto interaction = [&](int i, int b) -> double {
-        if constexpr (INT == INT_NONE) {
-            return 0.0;
-        }
-        else if constexpr (CHAIN) {
-            double um = 0.0, up = 0.0;
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                if (q == i) {
-                    um = (q == 0) ? left_of(b) : u[q > 0 ? q - 1 : 0];
-                    up = (q == cnt - 1) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
-                }
-            }
-            return chain_force<INT>(S, um, u[i], up);
-        }
-        else if constexpr (INT == INT_LONGRANGE1D) {
-            const double* U = mir + (int64_t)b * N;
-            return longrange_force(
-                S, [&](int64_t q) { return U[q]; }, [&](int64_t d) { return pref[d]; }, base + i);
-        }
-        else {
-            const double* U = mir + (int64_t)b * N;
-            const int64_t p = base + i;
-            return lattice_force<INT>(S, [&](int64_t q) { return U[q]; }, p / S.cols, p % S.cols);
-        }
-    };
-
-    // quasistatic driver ------------------------------------------------------
-    // block sums of (xi, xd) in a fixed order; two barriers
-    auto qs_sum = [&](long long xi, double xd, long long& ri, double& rd) __attribute__((always_inline)) {
-        xi = wave_sum_i(xi);
-        xd = wave_sum(xd);
-        if (lane == 0) {
-            s_qs_i[wave] = xi;
-            s_qs_d[wave] = xd;
-        }
-        __syncthreads();
-        long long ti = 0;
-        double td = 0.0;
-        for (int w = 0; w < nw; ++w) {
-            ti += s_qs_i[w];
-            td += s_qs_d[w];
-        }
-        __syncthreads();
-        ri = uniform_ll(ti);
-        rd = uniform_d(td);
-    };
-
-    // block minimum of xd and OR of xi; two barriers
-    auto qs_min = [&](long long xi, double xd, long long& ri, double& rd) __attribute__((always_inline)) {
-        for (int off = 32; off > 0; off >>= 1) {
-            const double y = __shfl_xor(xd, off, 64);
-            xd = y < xd ? y : xd;
-            xi |= __shfl_xor(xi, off, 64);
-        }
-        if (lane == 0) {
-            s_qs_i[wave] = xi;
-            s_qs_d[wave] = xd;
-        }
-        __syncthreads();
-        long long ti = s_qs_i[0];
-        double td = s_qs_d[0];
-        for (int w = 1; w < nw; ++w) {
-            ti |= s_qs_i[w];
-            td = s_qs_d[w] < td ? s_qs_d[w] : td;
-        }
-        __syncthreads();
-        ri = uniform_ll(ti);
-        rd = uniform_d(td);
-    };
-
-    // advanceUniformly(du, input_is_frame = false): u += du, u_frame += du
-    // (k_frame + mu) / k_frame, and each particle's well found by a walk inside
-    // its window; returns true if some particle left the window (the state is
-    // then left for k_refill to align at u)
-    auto qs_advance = [&](double du) __attribute__((always_inline)) -> bool {
-        long long need = 0;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                u[i] = u[i] + du;
-                const double x = u[i];
-                if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
-                    const double* row = Y + (base + i) * nch;
-                    int32_t j = jl[i];
-                    double lo = yl[i], hi = yr[i];
-                    while (x >= hi && j < jmax) {
-                        ++j;
-                        lo = hi;
-                        hi = row[j + 1];
-                    }
-                    while (x < lo && j > jmin) {
-                        --j;
-                        hi = lo;
-                        lo = row[j];
-                    }
-                    if (lo <= x && x < hi) {
-                        dl[i] += j - jl[i];
-                        jl[i] = j;
-                        yl[i] = lo;
-                        yr[i] = hi;
-                    }
-                    else {
-                        need = 1;
-                    }
-                }
-            }
-        }
-        uf = uf + du * (kf + S.mu) / kf;
-        long long any;
-        double unused;
-        qs_sum(need, 0.0, any, unused);
-        return any != 0;
-    };
-
-    // From equilibrium (phase QS_RECORD) or an event's start (QS_BEGIN): record
-    // finished events and apply the displacements of the next ones until a
-    // minimisation starts (ST_RUNNING, counters reset), the run is complete
-    // (ST_DONE) or a displacement needs a refill (ST_REFILL).
-    auto qs_next = [&]() __attribute__((always_inline)) -> int32_t {
-        for (;;) {
-            if (phase == QS_RECORD) {
-                long long sd = 0;
-                double sfr = 0.0;
-#pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
-                        sd += dl[i];
-                        sfr += kf * (uf - u[i]);
-                    }
-                }
-                long long tot;
-                double tfr;
-                qs_sum(sd, sfr, tot, tfr);
-                if (t == 0) {
-                    const int64_t o = (int64_t)blockIdx.x * qa.nevents + ev;
-                    if (qa.out_S) {
-                        qa.out_S[o] = tot - isum;
-                    }
-                    if (qa.out_uf) {
-                        qa.out_uf[o] = uf;
-                    }
-                    if (qa.out_ff) {
-                        qa.out_ff[o] = tfr / (double)N;
-                    }
-                    if (qa.out_inc) {
-                        qa.out_inc[o] = inc;
-                    }
-                }
-                ++ev;
-                phase = QS_BEGIN;
-            }
-            if (ev >= qa.nevents) {
-                return ST_DONE;
-            }
-            // QS_BEGIN
-            {
-                long long sd = 0;
-#pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
-                        sd += dl[i];
-                    }
-                }
-                long long tot;
-                double unused;
-                qs_sum(sd, 0.0, tot, unused);
-                isum = tot;
-            }
-            const int64_t k = qa.first + ev;
-            bool move = false;
-            double du = 0.0;
-            if (k == 0) { // initial quench: u_frame = 0, then minimise
-                uf = 0.0;
-                phase = QS_MINIMISE;
-            }
-            else if ((k & 1) && !qa.kicks) { // elastic: eventDrivenStep(eps, false)
-                double m = __builtin_inf();
-                int outside = 0;
-#pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
-                        const double x = max_uniform_particle<POT>(S, u[i], yl[i], yr[i], qa.direction, outside);
-                        m = x < m ? x : m;
-                    }
-                }
-                long long out;
-                qs_min(outside, m, out, m);
-                if (out) {
-                    m = 0.0;
-                }
-                if (!(m < 0.5 * qa.eps)) {
-                    du = qa.direction > 0 ? m - 0.5 * qa.eps : 0.5 * qa.eps - m;
-                    move = true;
-                }
-                phase = QS_RECORD;
-            }
-            else { // kick: eventDrivenStep(eps, true), then minimise
-                du = qa.direction > 0 ? qa.eps : -qa.eps;
-                move = true;
-                phase = QS_MINIMISE;
-            }
-            if (move && qs_advance(du)) {
-                refill_u = 1;
-                return ST_REFILL;
-            }
-            if (phase == QS_MINIMISE) {
-                below = 0;
-                below2 = 0;
-                dec = 0;
-                prev = __builtin_inf();
-                ev_done0 = 0;
-                return ST_RUNNING;
-            }
-        }
-    };
-
-    int32_t status = ST_RUNNING;
-    int it = 1;
-    // quasistatic driver: a launch that starts outside a minimisation goes
-    // through the transition at its first iteration
-    bool qs_pending = QS && phase != QS_MINIMISE;
-    for (;;) { // the quasistatic driver's minimisations (one pass otherwise)
-        for (;; ++it) {
-            const int b = it & 1;
-            // A: publish the slips after the position update of step `it`, and
-            // check (read only) that every particle's new well is reachable inside
-            // its window; otherwise step `it` is left to a refill and a relaunch
-            publish(b, true);
-            {
-                int refill = 0;
-#pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
-                        const double x = u[i] + dt * v[i] + c2 * a[i];
-                        if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
-                            refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
-                        }
-                    }
-                }
-                if (refill) {
-                    s_refill[b] = it;
-                }
-            }
-            __syncthreads();
-
-            // B: decisions on step it-1
-            if (it > it0 && uniform_i(s_nan[(it - 1) & 1]) == it - 1) {
-                status = ST_NAN;
-                break;
-            }
-            if (track && it > it0) {
-                const int pb = (it - 1) & 1;
-                double sf = 0.0, sff = 0.0;
-                long long nchg = 0, sabs = 0;
-                for (int w = 0; w < nw; ++w) {
-                    sf += s_red_d[pb][w][0];
-                    sff += s_red_d[pb][w][1];
-                    nchg += s_red_i[pb][w][0];
-                    sabs += s_red_i[pb][w][1];
-                }
-                sf = uniform_d(sf);
-                sff = uniform_d(sff);
-                nchg = uniform_ll(nchg);
-                sabs = uniform_ll(sabs);
-                if (ra.mode == MODE_UNTIL_EVENT && nchg > 0) {
-                    status = ST_EVENT;
-                }
-                else {
-                    const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
-                    const double r = (rff != 0.0) ? rf / rff : rf;
-                    below = !(r >= ra.tol) ? below + 1 : 0;
-                    below2 = !(r >= ra.tol2) ? below2 + 1 : 0;
-                    dec = !(r >= prev) ? dec + 1 : 0;
-                    prev = r;
-                    if (ra.mode == MODE_TRUNCATE || (ra.mode == MODE_MINIMISE && ra.time_activity)) {
-                        if (sabs != s_n) {
-                            if (act_init) {
-                                act_init = 0;
-                                qs_first = inc;
-                            }
-                            qs_last = inc;
-                        }
-                        s_n = sabs;
-                    }
-                    const int64_t nt = ra.niter_tol;
-                    const bool conv = nt <= 0 || (dec >= nt - 1 && below >= nt) || below2 >= nt;
-                    if (conv) {
-                        status = ST_CONVERGED;
-                    }
-                    else if (ra.mode == MODE_TRUNCATE) {
-                        if (ra.A_truncate > 0 && nchg >= ra.A_truncate) {
-                            status = ST_TRUNCATED;
-                        }
-                        else if (ra.S_truncate > 0 && sabs >= ra.S_truncate) {
-                            status = ST_TRUNCATED;
-                        }
-                    }
-                    if constexpr (QS) {
-                        if (status == ST_RUNNING && ev_done0 + (it - it0) >= qa.max_iter) {
-                            status = ST_MAXITER;
-                        }
-                    }
-                }
-            }
-            if (status != ST_RUNNING || it > limit || qs_pending) {
-                break;
-            }
-            if (uniform_i(s_refill[b]) == it) {
-                status = ST_REFILL;
-                break;
-            }
-
-            // C: step `it`
-            inc += 1;
-            if (ra.flow) {
-                uf = uf + dv;
-            }
-            if constexpr (EXT) {
-                if (inc >= min_next) {
-                    int due = 0;
+        // Publish the slips that other threads read: the current ones (`next`
+        // false) or those after the position update of the coming step, computed
+        // without touching the registers.
+        auto publish = [&](int b, bool next) {
+            if constexpr (CHAIN) {
+                if (cnt > 0) {
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt && inc >= S.next[base + i]) {
-                            ++due;
-                        }
-                    }
-                    int total = 0;
-                    int rank = block_excl_scan(due, s_scan_w, total);
-                    if (due > 0) {
-                        uint64_t st = pcg_advance(rng, rng_inc, 2 * (int64_t)rank);
-#pragma unroll
-                        for (int i = 0; i < PPT; ++i) {
-                            if (i < cnt) {
-                                const int64_t p = base + i;
-                                int64_t nx = S.next[p];
-                                if (inc >= nx) {
-                                    fth[i] = draw_normal(st, rng_inc, S.mean, S.stddev);
-                                    S.next[p] = nx + S.dinc[p];
-                                }
+                        if (i == 0 || i == cnt - 1) {
+                            const double x = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
+                            if (i == 0) {
+                                h_first[b * T + t] = x;
+                            }
+                            if (i == cnt - 1) {
+                                h_last[b * T + t] = x;
                             }
                         }
                     }
-                    long long mn = 0x7fffffffffffffffLL;
-#pragma unroll
-                    for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt) {
-                            long long nx = S.next[base + i];
-                            mn = nx < mn ? nx : mn;
-                        }
-                    }
-                    rng = pcg_advance(rng, rng_inc, 2 * (int64_t)uniform_i(total));
-                    min_next = uniform_ll(block_min_i(mn, s_min_w));
                 }
             }
+            else {
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i < cnt) {
+                        mir[(int64_t)b * N + base + i] = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
+                    }
+                }
+            }
+        };
 
-            // position update and well search inside the window (phase A made
-            // sure that the walk stays inside it; a non-finite slip keeps its well)
-            int nanf = 0;
+        if constexpr (INT == INT_LONGRANGE1D) {
+            for (int64_t q = t; q < N; q += T) {
+                pref[q] = S.pref[q];
+            }
+        }
+        publish(0, false);
+        if (t == 0) {
+            s_nan[0] = 0;
+            s_nan[1] = 0;
+            s_refill[0] = 0;
+            s_refill[1] = 0;
+        }
+
+
+        if constexpr (EXT) {
+            long long mn = 0x7fffffffffffffffLL;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 if (i < cnt) {
-                    u[i] = u[i] + dt * v[i] + c2 * a[i];
+                    long long nx = S.next[base + i];
+                    mn = nx < mn ? nx : mn;
+                }
+            }
+            min_next = uniform_ll(block_min_i(mn, s_min_w));
+        }
+        __syncthreads();
+
+        auto left_of = [&](int b) -> double { return h_last[b * T + (t == 0 ? Tact - 1 : t - 1)]; };
+        auto right_of = [&](int b) -> double { return h_first[b * T + (t == Tact - 1 ? 0 : t + 1)]; };
+
+        // interaction force of local particle i, neighbours from exchange buffer b
+        auto interaction = [&](int i, int b) -> double {
+            if constexpr (INT == INT_NONE) {
+                return 0.0;
+            }
+            else if constexpr (CHAIN) {
+                double um = 0.0, up = 0.0;
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) {
+                    if (q == i) {
+                        um = (q == 0) ? left_of(b) : u[q > 0 ? q - 1 : 0];
+                        up = (q == cnt - 1) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
+                    }
+                }
+                return chain_force<INT>(S, um, u[i], up);
+            }
+            else if constexpr (INT == INT_LONGRANGE1D) {
+                const double* U = mir + (int64_t)b * N;
+                return longrange_force(
+                    S, [&](int64_t q) { return U[q]; }, [&](int64_t d) { return pref[d]; }, base + i);
+            }
+            else {
+                const double* U = mir + (int64_t)b * N;
+                const int64_t p = base + i;
+                return lattice_force<INT>(S, [&](int64_t q) { return U[q]; }, p / S.cols, p % S.cols);
+            }
+        };
+
+        // quasistatic driver ------------------------------------------------------
+        // block sums of (xi, xd) in a fixed order; two barriers
+        auto qs_sum = [&](long long xi, double xd, long long& ri, double& rd) __attribute__((always_inline)) {
+            xi = wave_sum_i(xi);
+            xd = wave_sum(xd);
+            if (lane == 0) {
+                s_qs_i[wave] = xi;
+                s_qs_d[wave] = xd;
+            }
+            __syncthreads();
+            long long ti = 0;
+            double td = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                ti += s_qs_i[w];
+                td += s_qs_d[w];
+            }
+            __syncthreads();
+            ri = uniform_ll(ti);
+            rd = uniform_d(td);
+        };
+
+        // block minimum of xd and OR of xi; two barriers
+        auto qs_min = [&](long long xi, double xd, long long& ri, double& rd) __attribute__((always_inline)) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const double y = __shfl_xor(xd, off, 64);
+                xd = y < xd ? y : xd;
+                xi |= __shfl_xor(xi, off, 64);
+            }
+            if (lane == 0) {
+                s_qs_i[wave] = xi;
+                s_qs_d[wave] = xd;
+            }
+            __syncthreads();
+            long long ti = s_qs_i[0];
+            double td = s_qs_d[0];
+            for (int w = 1; w < nw; ++w) {
+                ti |= s_qs_i[w];
+                td = s_qs_d[w] < td ? s_qs_d[w] : td;
+            }
+            __syncthreads();
+            ri = uniform_ll(ti);
+            rd = uniform_d(td);
+        };
+
+        // advanceUniformly(du, input_is_frame = false): u += du, u_frame += du
+        // (k_frame + mu) / k_frame, and each particle's well found by a walk inside
+        // its window; returns true if some particle left the window (the state is
+        // then left for k_refill to align at u)
+        auto qs_advance = [&](double du) __attribute__((always_inline)) -> bool {
+            long long need = 0;
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i < cnt) {
+                    u[i] = u[i] + du;
                     const double x = u[i];
-                    nanf |= __builtin_isnan(x);
-                    if (!(yl[i] <= x && x < yr[i])) {
+                    if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
                         const double* row = Y + (base + i) * nch;
                         int32_t j = jl[i];
                         double lo = yl[i], hi = yr[i];
@@ -729,139 +478,442 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                             yl[i] = lo;
                             yr[i] = hi;
                         }
+                        else {
+                            need = 1;
+                        }
                     }
                 }
             }
-            if (nanf) {
-                s_nan[b] = it;
-            }
+            uf = uf + du * (kf + S.mu) / kf;
+            long long any;
+            double unused;
+            qs_sum(need, 0.0, any, unused);
+            return any != 0;
+        };
 
-            double sf = 0.0, sff = 0.0;
-            int nchg = 0, sabs = 0;
+        // From equilibrium (phase QS_RECORD) or an event's start (QS_BEGIN): record
+        // finished events and apply the displacements of the next ones until a
+        // minimisation starts (ST_RUNNING, counters reset), the run is complete
+        // (ST_DONE) or a displacement needs a refill (ST_REFILL).
+        auto qs_next = [&]() __attribute__((always_inline)) -> int32_t {
+            for (;;) {
+                if (phase == QS_RECORD) {
+                    long long sd = 0;
+                    double sfr = 0.0;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        if (i < cnt) {
+                            sd += dl[i];
+                            sfr += kf * (uf - u[i]);
+                        }
+                    }
+                    long long tot;
+                    double tfr;
+                    qs_sum(sd, sfr, tot, tfr);
+                    if (t == 0) {
+                        const int64_t o = (int64_t)blockIdx.x * qa.nevents + ev;
+                        if (qa.out_S) {
+                            qa.out_S[o] = tot - isum;
+                        }
+                        if (qa.out_uf) {
+                            qa.out_uf[o] = uf;
+                        }
+                        if (qa.out_ff) {
+                            qa.out_ff[o] = tfr / (double)N;
+                        }
+                        if (qa.out_inc) {
+                            qa.out_inc[o] = inc;
+                        }
+                    }
+                    ++ev;
+                    phase = QS_BEGIN;
+                }
+                if (ev >= qa.nevents) {
+                    return ST_DONE;
+                }
+                // QS_BEGIN
+                {
+                    long long sd = 0;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        if (i < cnt) {
+                            sd += dl[i];
+                        }
+                    }
+                    long long tot;
+                    double unused;
+                    qs_sum(sd, 0.0, tot, unused);
+                    isum = tot;
+                }
+                const int64_t k = qa.first + ev;
+                bool move = false;
+                double du = 0.0;
+                if (k == 0) { // initial quench: u_frame = 0, then minimise
+                    uf = 0.0;
+                    phase = QS_MINIMISE;
+                }
+                else if ((k & 1) && !qa.kicks) { // elastic: eventDrivenStep(eps, false)
+                    double m = __builtin_inf();
+                    int outside = 0;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        if (i < cnt) {
+                            const double x = max_uniform_particle<POT>(S, u[i], yl[i], yr[i], qa.direction, outside);
+                            m = x < m ? x : m;
+                        }
+                    }
+                    long long out;
+                    qs_min(outside, m, out, m);
+                    if (out) {
+                        m = 0.0;
+                    }
+                    if (!(m < 0.5 * qa.eps)) {
+                        du = qa.direction > 0 ? m - 0.5 * qa.eps : 0.5 * qa.eps - m;
+                        move = true;
+                    }
+                    phase = QS_RECORD;
+                }
+                else { // kick: eventDrivenStep(eps, true), then minimise
+                    du = qa.direction > 0 ? qa.eps : -qa.eps;
+                    move = true;
+                    phase = QS_MINIMISE;
+                }
+                if (phase == QS_MINIMISE) {
+                    below = 0;
+                    below2 = 0;
+                    dec = 0;
+                    prev = __builtin_inf();
+                    ev_done0 = 0;
+                }
+                if (move && qs_advance(du)) {
+                    refill_u = 1;
+                    return ST_REFILL;
+                }
+                if (phase == QS_MINIMISE) {
+                    return ST_RUNNING;
+                }
+            }
+        };
+
+        status = ST_RUNNING;
+        it = 1;
+        it0 = 1;
+        // quasistatic driver: an entry that starts outside a minimisation goes
+        // through the transition at its first iteration
+        bool qs_pending = QS && phase != QS_MINIMISE;
+        for (;;) { // the quasistatic driver's minimisations (one pass otherwise)
+            for (;; ++it) {
+                const int b = it & 1;
+                // A: publish the slips after the position update of step `it`, and
+                // check (read only) that every particle's new well is reachable inside
+                // its window; otherwise step `it` is left to a refill and a relaunch
+                publish(b, true);
+                {
+                    int refill = 0;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        if (i < cnt) {
+                            const double x = u[i] + dt * v[i] + c2 * a[i];
+                            if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
+                                refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
+                            }
+                        }
+                    }
+                    if (refill) {
+                        s_refill[b] = it;
+                    }
+                }
+                __syncthreads();
+
+                // B: decisions on step it-1
+                if (it > it0 && uniform_i(s_nan[(it - 1) & 1]) == it - 1) {
+                    status = ST_NAN;
+                    break;
+                }
+                if (track && it > it0) {
+                    const int pb = (it - 1) & 1;
+                    double sf = 0.0, sff = 0.0;
+                    long long nchg = 0, sabs = 0;
+                    for (int w = 0; w < nw; ++w) {
+                        sf += s_red_d[pb][w][0];
+                        sff += s_red_d[pb][w][1];
+                        nchg += s_red_i[pb][w][0];
+                        sabs += s_red_i[pb][w][1];
+                    }
+                    sf = uniform_d(sf);
+                    sff = uniform_d(sff);
+                    nchg = uniform_ll(nchg);
+                    sabs = uniform_ll(sabs);
+                    if (ra.mode == MODE_UNTIL_EVENT && nchg > 0) {
+                        status = ST_EVENT;
+                    }
+                    else {
+                        const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
+                        const double r = (rff != 0.0) ? rf / rff : rf;
+                        below = !(r >= ra.tol) ? below + 1 : 0;
+                        below2 = !(r >= ra.tol2) ? below2 + 1 : 0;
+                        dec = !(r >= prev) ? dec + 1 : 0;
+                        prev = r;
+                        if (ra.mode == MODE_TRUNCATE || (ra.mode == MODE_MINIMISE && ra.time_activity)) {
+                            if (sabs != s_n) {
+                                if (act_init) {
+                                    act_init = 0;
+                                    qs_first = inc;
+                                }
+                                qs_last = inc;
+                            }
+                            s_n = sabs;
+                        }
+                        const int64_t nt = ra.niter_tol;
+                        const bool conv = nt <= 0 || (dec >= nt - 1 && below >= nt) || below2 >= nt;
+                        if (conv) {
+                            status = ST_CONVERGED;
+                        }
+                        else if (ra.mode == MODE_TRUNCATE) {
+                            if (ra.A_truncate > 0 && nchg >= ra.A_truncate) {
+                                status = ST_TRUNCATED;
+                            }
+                            else if (ra.S_truncate > 0 && sabs >= ra.S_truncate) {
+                                status = ST_TRUNCATED;
+                            }
+                        }
+                        if constexpr (QS) {
+                            if (status == ST_RUNNING && ev_done0 + (it - it0) >= qa.max_iter) {
+                                status = ST_MAXITER;
+                            }
+                        }
+                    }
+                }
+                if (status != ST_RUNNING || it > limit || qs_pending) {
+                    break;
+                }
+                if (uniform_i(s_refill[b]) == it) {
+                    status = ST_REFILL;
+                    break;
+                }
+
+                // C: step `it`
+                inc += 1;
+                if (ra.flow) {
+                    uf = uf + dv;
+                }
+                if constexpr (EXT) {
+                    if (inc >= min_next) {
+                        int due = 0;
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) {
+                            if (i < cnt && inc >= S.next[base + i]) {
+                                ++due;
+                            }
+                        }
+                        int total = 0;
+                        int rank = block_excl_scan(due, s_scan_w, total);
+                        if (due > 0) {
+                            uint64_t st = pcg_advance(rng, rng_inc, 2 * (int64_t)rank);
+#pragma unroll
+                            for (int i = 0; i < PPT; ++i) {
+                                if (i < cnt) {
+                                    const int64_t p = base + i;
+                                    int64_t nx = S.next[p];
+                                    if (inc >= nx) {
+                                        fth[i] = draw_normal(st, rng_inc, S.mean, S.stddev);
+                                        S.next[p] = nx + S.dinc[p];
+                                    }
+                                }
+                            }
+                        }
+                        long long mn = 0x7fffffffffffffffLL;
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) {
+                            if (i < cnt) {
+                                long long nx = S.next[base + i];
+                                mn = nx < mn ? nx : mn;
+                            }
+                        }
+                        rng = pcg_advance(rng, rng_inc, 2 * (int64_t)uniform_i(total));
+                        min_next = uniform_ll(block_min_i(mn, s_min_w));
+                    }
+                }
+
+                // position update and well search inside the window (phase A made
+                // sure that the walk stays inside it; a non-finite slip keeps its well)
+                int nanf = 0;
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i < cnt) {
+                        u[i] = u[i] + dt * v[i] + c2 * a[i];
+                        const double x = u[i];
+                        nanf |= __builtin_isnan(x);
+                        if (!(yl[i] <= x && x < yr[i])) {
+                            const double* row = Y + (base + i) * nch;
+                            int32_t j = jl[i];
+                            double lo = yl[i], hi = yr[i];
+                            while (x >= hi && j < jmax) {
+                                ++j;
+                                lo = hi;
+                                hi = row[j + 1];
+                            }
+                            while (x < lo && j > jmin) {
+                                --j;
+                                hi = lo;
+                                lo = row[j];
+                            }
+                            if (lo <= x && x < hi) {
+                                dl[i] += j - jl[i];
+                                jl[i] = j;
+                                yl[i] = lo;
+                                yr[i] = hi;
+                            }
+                        }
+                    }
+                }
+                if (nanf) {
+                    s_nan[b] = it;
+                }
+
+                double sf = 0.0, sff = 0.0;
+                int nchg = 0, sabs = 0;
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    if (i < cnt) {
+                        const double fi = interaction(i, b);
+                        const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
+                        const double ff = kf * (uf - u[i]);
+                        const double sum3 = ff + fp + fi;
+                        const double vn = v[i], an = a[i];
+                        double vv = vn + dt * an;
+                        double f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                        double aa = f * inv_m;
+                        vv = vn + hdt * (an + aa);
+                        f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                        aa = f * inv_m;
+                        vv = vn + hdt * (an + aa);
+                        f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
+                        aa = f * inv_m;
+                        v[i] = vv;
+                        a[i] = aa;
+                        if (track) {
+                            sf += f * f;
+                            sff += ff * ff;
+                            nchg += dl[i] != 0;
+                            sabs += dl[i] < 0 ? -dl[i] : dl[i];
+                        }
+                    }
+                }
+                if (track) {
+                    sf = wave_sum(sf);
+                    sff = wave_sum(sff);
+                    long long n1 = wave_sum_i(nchg), n2 = wave_sum_i(sabs);
+                    if (lane == 0) {
+                        s_red_d[b][wave][0] = sf;
+                        s_red_d[b][wave][1] = sff;
+                        s_red_i[b][wave][0] = n1;
+                        s_red_i[b][wave][1] = n2;
+                    }
+                }
+            }
+            if constexpr (QS) {
+                if (status == ST_CONVERGED || qs_pending) {
+                    if (!qs_pending) {
+                        // quench (detail.h:1782), then record the event
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) {
+                            v[i] = 0.0;
+                            a[i] = 0.0;
+                        }
+                        phase = QS_RECORD;
+                    }
+                    qs_pending = false;
+                    status = qs_next();
+                    if (phase == QS_MINIMISE) {
+                        it0 = it; // a new minimisation starts at iteration `it`
+                    }
+                    if (status == ST_RUNNING) {
+                        // phase A's speculative slips of iteration `it` are stale
+                        if (t == 0) {
+                            s_refill[it & 1] = 0;
+                        }
+                        __syncthreads();
+                        continue;
+                    }
+                }
+            }
+            break;
+        }
+
+        const int done = it - 1; // completed steps
+        if (status == ST_CONVERGED && !QS) {
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                if (i < cnt) {
-                    const double fi = interaction(i, b);
+                v[i] = 0.0;
+                a[i] = 0.0;
+            }
+        }
+        if (status == ST_RUNNING) {
+            status = ST_BUDGET;
+        }
+
+        // write back the state and the forces of the final configuration; the
+        // exchange buffer done&1 holds its slips (published at the start of the
+        // iteration after the last completed step, or initially), except after a
+        // displacement of the quasistatic driver: publish them again
+        const int fb = done & 1;
+        const bool forces = status != ST_NAN && status != ST_REFILL;
+        if constexpr (QS) {
+            if (forces) {
+                publish(fb, false);
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t p = base + i;
+                S.u[p] = u[i];
+                S.v[p] = v[i];
+                S.a[p] = a[i];
+                L.jloc[p] = jl[i];
+                if (forces) {
+                    const double fi = interaction(i, fb);
                     const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
                     const double ff = kf * (uf - u[i]);
-                    const double sum3 = ff + fp + fi;
-                    const double vn = v[i], an = a[i];
-                    double vv = vn + dt * an;
-                    double f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                    double aa = f * inv_m;
-                    vv = vn + hdt * (an + aa);
-                    f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                    aa = f * inv_m;
-                    vv = vn + hdt * (an + aa);
-                    f = EXT ? sum3 + meta * vv + fth[i] : sum3 + meta * vv;
-                    aa = f * inv_m;
-                    v[i] = vv;
-                    a[i] = aa;
-                    if (track) {
-                        sf += f * f;
-                        sff += ff * ff;
-                        nchg += dl[i] != 0;
-                        sabs += dl[i] < 0 ? -dl[i] : dl[i];
-                    }
+                    const double fd = meta * v[i];
+                    S.f_int[p] = fi;
+                    S.f_pot[p] = fp;
+                    S.f_frame[p] = ff;
+                    S.f_damp[p] = fd;
+                    S.f[p] = EXT ? ff + fp + fi + fd + fth[i] : ff + fp + fi + fd;
                 }
-            }
-            if (track) {
-                sf = wave_sum(sf);
-                sff = wave_sum(sff);
-                long long n1 = wave_sum_i(nchg), n2 = wave_sum_i(sabs);
-                if (lane == 0) {
-                    s_red_d[b][wave][0] = sf;
-                    s_red_d[b][wave][1] = sff;
-                    s_red_i[b][wave][0] = n1;
-                    s_red_i[b][wave][1] = n2;
+                if constexpr (EXT) {
+                    S.f_th[p] = fth[i];
                 }
             }
         }
         if constexpr (QS) {
-            if (status == ST_CONVERGED || qs_pending) {
-                if (!qs_pending) {
-                    // quench (detail.h:1782), then record the event
-#pragma unroll
-                    for (int i = 0; i < PPT; ++i) {
-                        v[i] = 0.0;
-                        a[i] = 0.0;
-                    }
-                    phase = QS_RECORD;
-                }
-                qs_pending = false;
-                status = qs_next();
-                if (status == ST_RUNNING) {
-                    // a new minimisation starts from the displaced state at
-                    // iteration `it` (phase A's speculative slips are stale)
-                    it0 = it;
-                    if (t == 0) {
-                        s_refill[it & 1] = 0;
-                    }
-                    __syncthreads();
-                    continue;
-                }
-            }
+            ev_done0 = phase == QS_MINIMISE ? ev_done0 + (it - it0) : 0;
         }
-        break;
-    }
-
-    const int done = it - 1; // completed steps
-    if (status == ST_CONVERGED && !QS) {
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            v[i] = 0.0;
-            a[i] = 0.0;
+        launch_done += done;
+        if (status != ST_REFILL) {
+            break;
         }
-    }
-    if (status == ST_RUNNING) {
-        status = ST_BUDGET;
-    }
-
-    // write back the state and the forces of the final configuration; the
-    // exchange buffer done&1 holds its slips (published at the start of the
-    // iteration after the last completed step, or initially), except after a
-    // displacement of the quasistatic driver: publish them again
-    const int fb = done & 1;
-    if constexpr (QS) {
-        publish(fb, false);
+        // refill from the written-back state, then reload it: the LDS of this
+        // entry must no longer be read when the next one publishes
+        refill_own(views, base, cnt, refill_u);
+        refill_u = 0;
         __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        if (i < cnt) {
-            const int64_t p = base + i;
-            S.u[p] = u[i];
-            S.v[p] = v[i];
-            S.a[p] = a[i];
-            L.jloc[p] = jl[i];
-            if (status != ST_NAN) {
-                const double fi = interaction(i, fb);
-                const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
-                const double ff = kf * (uf - u[i]);
-                const double fd = meta * v[i];
-                S.f_int[p] = fi;
-                S.f_pot[p] = fp;
-                S.f_frame[p] = ff;
-                S.f_damp[p] = fd;
-                S.f[p] = EXT ? ff + fp + fi + fd + fth[i] : ff + fp + fi + fd;
-            }
-            if constexpr (EXT) {
-                S.f_th[p] = fth[i];
-            }
-        }
-    }
+
     if (t == 0) {
         rs->inc = inc;
         rs->u_frame = uf;
         if constexpr (QS) {
-            rs->steps = phase == QS_MINIMISE ? ev_done0 + (it - it0) : 0;
+            rs->steps = ev_done0;
             rs->qs_event = ev;
             rs->qs_phase = phase;
             rs->qs_isum = isum;
-            rs->refill_u = refill_u;
         }
         else {
-            rs->steps += done;
+            rs->steps += launch_done;
         }
         rs->status = status;
         rs->below = below;

@@ -235,12 +235,10 @@ __global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double e
     S.u[p] = direction > 0 ? yr + 0.5 * eps : yl - 0.5 * eps;
 }
 
-// Landscape refill requested by a resident launch (status ST_REFILL): align
-// every particle to its slip after the position update of the pending step,
-// u + dt v + dt^2/2 a (bit-identical to the resident kernel), or, with
-// rs->refill_u set (a uniform displacement of the quasistatic driver), to u
-// itself, shifting and regenerating its window like the CPU align; then the
-// run continues.
+// Landscape refill requested by a launch (status ST_REFILL): align every
+// particle to its slip after the position update of the pending step,
+// u + dt v + dt^2/2 a (bit-identical to the launching kernel), shifting and
+// regenerating its window like the CPU align; then the run continues.
 __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restrict__ views)
 {
     const SysView& S = views[blockIdx.x];
@@ -251,9 +249,8 @@ __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restric
     const Landscape& L = S.ls;
     const double dt = S.dt;
     const double c2 = 0.5 * dt * dt;
-    const bool at_u = rs->refill_u != 0;
     for (int64_t p = threadIdx.x; p < S.N; p += blockDim.x) {
-        const double x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
+        const double x = S.u[p] + dt * S.v[p] + c2 * S.a[p];
         const int32_t j = L.jloc[p];
         const double* row = L.y + p * L.nchunk;
         if (!(row[j] <= x && x < row[j + 1])) {
@@ -263,7 +260,6 @@ __global__ void __launch_bounds__(SVC_THREADS) k_refill(const SysView* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
         rs->status = ST_RUNNING;
-        rs->refill_u = 0;
     }
 }
 

@@ -260,6 +260,10 @@ int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direct
 int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
                            double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
                            double* f_frame, int64_t* inc);
+/* The last fqsb_batch_quasistatic: time between HIP events on the launch
+ * stream around its device work (ms), and the number of driver launches (0
+ * if it was driven event by event from the host). */
+int fqsb_batch_quasistatic_stats(const fqsb_batch* b, float* ms, int64_t* launches);
 /* Arrays of all replicas, [R, N]; scalars [R]. */
 int fqsb_batch_get(fqsb_batch* b, int field, double* out);
 int fqsb_batch_set(fqsb_batch* b, int field, const double* in);
