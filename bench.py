@@ -623,23 +623,26 @@ def run_ragged(args):
     nps = ragged_nopassing(systems)
     rgn = Ragged(nps)
     nsizes = np.array(rgn.sizes, dtype=float)
-    rg.minimise()
+
+    def events(first, k):
+        """k event pairs (elastic step, kick + minimise) from protocol event
+        `first` on the systems with maxUniformDisplacement, k kicks + minimise
+        on the Smooth ones (whose maxUniformDisplacement raises)"""
+        rg.quasistatic(RG_EPS, 2 * k, first_event=first, active=~smooth)
+        rg.quasistatic(RG_EPS, k, first_event=(first + 1) // 2, kicks_only=True, active=smooth)
+
+    rg.quasistatic(RG_EPS, 1)  # event 0: the initial quench
     rgn.minimise()
-    for _ in range(args.warmup):
-        rg.eventDrivenStep(RG_EPS, False, active=~smooth)
-        rg.eventDrivenStep(RG_EPS, True)
-        rg.minimise()
+    if args.warmup:
+        events(1, args.warmup)
+    first = 1 + 2 * args.warmup
+    inc0 = np.array(rg.inc, dtype=float)
     torch.cuda.synchronize()
-    work = 0.0
-    events = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rg.eventDrivenStep(RG_EPS, False, active=~smooth)
-        rg.eventDrivenStep(RG_EPS, True)
-        rg.minimise()
-        work += float(np.sum(sizes * rg.last_steps))
-        events += 1
+    events(first, args.steps)
+    torch.cuda.synchronize()
     t_inertial = time.perf_counter() - t0
+    work = float(np.sum(sizes * (np.array(rg.inc, dtype=float) - inc0)))
     for _ in range(args.steps):
         rgn.eventDrivenStep(RG_EPS, False)
         rgn.eventDrivenStep(RG_EPS, True)
@@ -663,7 +666,8 @@ def run_ragged(args):
         "data": "synthetic (Weibull yield landscapes from pcg32, disjoint seeds per system)",
         "config": {
             "workload": f"ragged: {len(systems)} systems ({int(sizes.sum())} particles, {rg.groups} launch groups), "
-            f"{args.steps} events each, then {args.steps} no-passing events on {len(nps)} Cuspy-Laplace chains",
+            f"{args.steps} events each (device-side quasistatic driver), then {args.steps} no-passing events on "
+            f"{len(nps)} Cuspy-Laplace chains",
             "systems": len(systems),
             "particles": int(sizes.sum()),
             "groups": rg.groups,
@@ -755,7 +759,7 @@ def main():
         "--steps", type=int, default=None,
         help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
         "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5; "
-        "ragged: quasistatic events, default 20; quasistatic: event pairs, default 100)",
+        "ragged: quasistatic events, default 50; quasistatic: event pairs, default 100)",
     )
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -784,7 +788,7 @@ def main():
         )
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 20, "quasistatic": 100}[args.workload]
+        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 50, "quasistatic": 100}[args.workload]
     run = {
         "flow": run_flow,
         "longrange": run_longrange,

@@ -176,25 +176,41 @@ FQ_HD size_t resident_lds_bytes(int64_t N, int threads)
     }
 }
 
-// Landscape refill inside a resident launch, by the thread that owns
-// particles base .. base + cnt - 1 (only it reads their rows): align each to
-// the slip of the pending step, u + dt v + dt^2/2 a, or to u itself (at_u, a
-// uniform displacement of the quasistatic driver), from the state in memory,
-// exactly as k_refill does. Called with the state written back, so that no
-// registers are live across the call.
-__device__ __noinline__ void refill_own(const SysView* __restrict__ views, int64_t base, int cnt, int at_u)
+// Landscape refill inside a resident launch: every particle p = base ..
+// base + cnt - 1 of every lane is aligned to the slip of the pending step,
+// u + dt v + dt^2/2 a, or to u itself (at_u, a uniform displacement of the
+// quasistatic driver), from the state in memory, exactly as k_refill does.
+// The particles that leave their well are taken one at a time by the whole
+// wavefront (chunk_align_wave: the new yield positions are drawn in
+// parallel). Called with the state written back, so that no registers are
+// live across the call.
+__device__ __noinline__ void refill_wave(const SysView* __restrict__ views, int64_t base, int cnt, int at_u)
 {
     const SysView& S = views[blockIdx.x];
     const Landscape& L = S.ls;
     const double dt = S.dt;
     const double c2 = 0.5 * dt * dt;
-    for (int i = 0; i < cnt; ++i) {
+    int cmax = cnt; // the most particles a lane of this wavefront holds
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(cmax, off, 64);
+        cmax = o > cmax ? o : cmax;
+    }
+    for (int i = 0; i < cmax; ++i) {
         const int64_t p = base + i;
-        const double x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
-        const int32_t j = L.jloc[p];
-        const double* row = L.y + p * L.nchunk;
-        if (!(row[j] <= x && x < row[j + 1])) {
-            chunk_align_slow(L, p, x, j);
+        double x = 0.0;
+        int32_t j = 0;
+        bool need = false;
+        if (i < cnt) {
+            x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
+            j = L.jloc[p];
+            const double* row = L.y + p * L.nchunk;
+            need = !(row[j] <= x && x < row[j + 1]);
+        }
+        uint64_t pending = __ballot(need);
+        while (pending) {
+            const int src = __builtin_ctzll(pending);
+            pending &= pending - 1;
+            chunk_align_wave(L, __shfl(p, src, 64), __shfl(x, src, 64), __shfl(j, src, 64));
         }
     }
 }
@@ -207,7 +223,7 @@ __device__ __noinline__ void refill_own(const SysView* __restrict__ views, int64
 // same iteration, all inside the workgroup; it leaves the launch only for the
 // step budget, a NaN, max_iter or the end of the run.
 // A step (or displacement) that needs a landscape refill writes the state
-// back, refills (refill_own) and re-enters with the state reloaded.
+// back, refills (refill_wave) and re-enters with the state reloaded.
 template <int POT, int INT, int EXT, int PPT, int MAXT, bool QS>
 __device__ __forceinline__ void resident_body(const SysView* __restrict__ views, const RunArgs& ra, const QsArgs& qa)
 {
@@ -898,7 +914,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         }
         // refill from the written-back state, then reload it: the LDS of this
         // entry must no longer be read when the next one publishes
-        refill_own(views, base, cnt, refill_u);
+        refill_wave(views, base, cnt, refill_u);
         refill_u = 0;
         __syncthreads();
     }
@@ -938,7 +954,8 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
 // Quasistatic driver: the events of QsArgs for every system, one workgroup
 // per system, relaunched after refills and exhausted step budgets.
 template <int POT, int INT, int PPT, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_quasi(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT <= 256 && PPT <= 8 ? 2 : 1)))
+k_quasi(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 {
     resident_body<POT, INT, 0, PPT, MAXT, true>(views, ra, qa);
 }
