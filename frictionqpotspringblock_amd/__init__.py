@@ -1,7 +1,7 @@
 """MI355X (gfx950) implementation of FrictionQPotSpringBlock's dynamics hot path.
 
 Drop-in for the reference's Python module ``FrictionQPotSpringBlock``: the
-submodules ``Line1d``, ``Line2d`` and ``detail`` expose the same classes,
+submodules ``Line1d``, ``Line2d``, ``Particles`` and ``detail`` expose the same classes,
 argument names and defaults (python/main.cpp:228-1006). Every computation runs
 in the HIP library ``libfqsb.so`` through its C ABI (include/fqsb.h); there is
 no CPU fallback, and importing fails if the native library is not built.
@@ -24,13 +24,15 @@ except ImportError as e:  # pragma: no cover - exercised only without a build
 
 Line1d = _fqsb.Line1d
 Line2d = _fqsb.Line2d
+Particles = _fqsb.Particles
 detail = _fqsb.detail
 version = _fqsb.version
 
 sys.modules[__name__ + ".Line1d"] = Line1d
 sys.modules[__name__ + ".Line2d"] = Line2d
+sys.modules[__name__ + ".Particles"] = Particles
 sys.modules[__name__ + ".detail"] = detail
 
 LIBRARY_PATH = os.path.join(_here, "libfqsb.so")
 
-__all__ = ["Line1d", "Line2d", "detail", "version", "LIBRARY_PATH"]
+__all__ = ["Line1d", "Line2d", "Particles", "detail", "version", "LIBRARY_PATH"]

@@ -163,6 +163,12 @@ SIGNATURES = {
          _ip],
     ),
     "fqsb_batch_quasistatic_stats": (C.c_int, [_P, _fp, _ip]),
+    "fqsb_batch_minimise_truncate": (
+        C.c_int,
+        [_P, _ip, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up],
+    ),
+    "fqsb_batch_trigger": (C.c_int, [_P, _up, C.c_double, C.c_int]),
+    "fqsb_batch_advance_to_fixed_force": (C.c_int, [_P, _dp, C.c_int]),
     "fqsb_quasistatic": (
         C.c_int,
         [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
@@ -459,6 +465,29 @@ class Batch:
             )
         )
         return ret
+
+    def minimise_truncate(self, i_n, A_truncate=0, S_truncate=0, tol=1e-5, niter_tol=10, max_iter=int(1e9),
+                          max_iter_is_error=True):
+        """minimise_truncate (detail.h:1833-1893) of every replica; i_n [R, N]."""
+        i_n = np.ascontiguousarray(i_n, dtype=np.int64).reshape(self._R, self._N)
+        ret = np.zeros(self._R, dtype=np.uint64)
+        _check(
+            capi().fqsb_batch_minimise_truncate(
+                self._h, i_n.ctypes.data_as(_ip), int(A_truncate), int(S_truncate), float(tol), int(niter_tol),
+                int(max_iter), int(bool(max_iter_is_error)), ret.ctypes.data_as(_up),
+            )
+        )
+        return ret
+
+    def trigger(self, p, eps, direction=1):
+        """trigger (detail.h:1972-1977) of particle p[r] in replica r."""
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, dtype=np.uint64), [self._R]))
+        _check(capi().fqsb_batch_trigger(self._h, p.ctypes.data_as(_up), float(eps), int(direction)))
+
+    def advanceToFixedForce(self, f_frame, allow_plastic=False):
+        """advanceToFixedForce (detail.h:1988-1995) to mean frame force f_frame[r]."""
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(f_frame, dtype=float), [self._R]))
+        _check(capi().fqsb_batch_advance_to_fixed_force(self._h, f.ctypes.data_as(_dp), int(bool(allow_plastic))))
 
     def eventDrivenStep(self, eps, kick, direction=1):
         ret = np.zeros(self._R)

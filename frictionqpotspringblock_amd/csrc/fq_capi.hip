@@ -1061,6 +1061,97 @@ struct Group {
         return ret;
     }
 
+    // minimise_truncate detail.h:1833-1893 for every system from the reference
+    // indices i_n (all systems' particles, system after system); ret[r] as
+    // System::minimise_truncate
+    void minimise_truncate(const int64_t* i_n, uint64_t A_truncate, uint64_t S_truncate, double tol,
+                           uint64_t niter_tol, uint64_t max_iter, bool max_iter_is_error, uint64_t* ret)
+    {
+        if (minim != FQSB_MIN_INERTIAL) {
+            throw Error("minimise_truncate is not available for this system");
+        }
+        h2d(i_ref, i_n, (size_t)total);
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_TRUNCATE;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        ra.A_truncate = (int64_t)A_truncate;
+        ra.S_truncate = (int64_t)S_truncate;
+        run(ra, max_iter, MIN_BLOCK, false);
+        for (int64_t k = 0; k < R; ++k) {
+            const auto& r = hrs[(size_t)k];
+            if (r.status == fq::ST_NAN) {
+                throw Error("NaN entries found");
+            }
+            if (r.status == fq::ST_CONVERGED) {
+                ret[k] = 0;
+            }
+            else if (r.status == fq::ST_TRUNCATED) {
+                ret[k] = (uint64_t)r.steps;
+            }
+            else if (max_iter_is_error) {
+                throw Error("No convergence found");
+            }
+            else {
+                ret[k] = max_iter + 1;
+            }
+        }
+    }
+
+    // advanceToFixedForce detail.h:1988-1995 for every system: target[r] is
+    // the wanted mean frame force of system r (mean taken in particle order)
+    void advance_to_fixed_force(const double* target, bool allow_plastic)
+    {
+        const size_t T = (size_t)total;
+        std::vector<int64_t> st(T), st2(T);
+        std::vector<int32_t> jl(T), jl2(T);
+        std::vector<double> ff(T);
+        d2h(st.data(), start, T);
+        d2h(jl.data(), jloc, T);
+        d2h(ff.data(), f_frame, T);
+        std::vector<double> du((size_t)R);
+        for (int64_t r = 0; r < R; ++r) {
+            double mean = 0.0;
+            for (int64_t p = off[(size_t)r]; p < off[(size_t)r + 1]; ++p) {
+                mean += ff[(size_t)p];
+            }
+            mean /= (double)(off[(size_t)r + 1] - off[(size_t)r]);
+            du[(size_t)r] = (target[r] - mean) / hviews[(size_t)r].mu;
+        }
+        advance_uniformly(du, nullptr);
+        d2h(st2.data(), start, T);
+        d2h(jl2.data(), jloc, T);
+        bool same = true;
+        for (size_t p = 0; p < T; ++p) {
+            same &= st[p] + jl[p] == st2[p] + jl2[p];
+        }
+        if (!allow_plastic && !same) {
+            throw Error("assertion failed (allow_plastic || xt::all(xt::equal(m_chunk->index_at_align(), i_n)))");
+        }
+    }
+
+    // trigger detail.h:1972-1977 of particle p[r] in every system r (forces
+    // are left as they are, as in the reference)
+    void trigger_each(const uint64_t* p, double eps, int direction)
+    {
+        std::vector<int64_t> hp((size_t)R);
+        for (int64_t r = 0; r < R; ++r) {
+            if ((int64_t)p[r] >= off[(size_t)r + 1] - off[(size_t)r]) {
+                throw Error("particle index out of range");
+            }
+            hp[(size_t)r] = (int64_t)p[r];
+        }
+        int64_t* dp = dalloc<int64_t>((size_t)R);
+        h2d(dp, hp.data(), (size_t)R);
+        const hipError_t e =
+            fq::launch_trigger_each(views, (int)R, dp, eps, direction, pot == FQSB_POT_CUSPY ? 1 : 0, stream);
+        const hipError_t e2 = hipStreamSynchronize(stream);
+        dfree(dp);
+        HIPC(e);
+        HIPC(e2);
+    }
+
     // The quasistatic protocol of the reference's examples
     // (examples/Line1d_Cuspy_Laplace.py:39-61) for every system (with mask
     // set: the masked-in ones): events k = first .. first + nevents - 1,
@@ -2043,35 +2134,7 @@ int fqsb_minimise_truncate(fqsb_system* s, const int64_t* i_n, uint64_t A_trunca
 {
     (void)time_activity;
     return guard([&] {
-        Group& g = *s->g;
-        if (g.minim != FQSB_MIN_INERTIAL) {
-            throw Error("minimise_truncate is not available for this system");
-        }
-        g.h2d(g.i_ref, i_n, (size_t)g.N);
-        fq::RunArgs ra{};
-        ra.mode = fq::MODE_TRUNCATE;
-        ra.tol = tol;
-        ra.tol2 = tol * tol;
-        ra.niter_tol = (int64_t)niter_tol;
-        ra.A_truncate = (int64_t)A_truncate;
-        ra.S_truncate = (int64_t)S_truncate;
-        g.run(ra, max_iter, MIN_BLOCK, false);
-        const auto& r = g.hrs[0];
-        if (r.status == fq::ST_NAN) {
-            throw Error("NaN entries found");
-        }
-        if (r.status == fq::ST_CONVERGED) {
-            *ret = 0;
-            return;
-        }
-        if (r.status == fq::ST_TRUNCATED) {
-            *ret = (uint64_t)r.steps;
-            return;
-        }
-        if (max_iter_is_error) {
-            throw Error("No convergence found");
-        }
-        *ret = max_iter + 1;
+        s->g->minimise_truncate(i_n, A_truncate, S_truncate, tol, niter_tol, max_iter, max_iter_is_error != 0, ret);
     });
 }
 
@@ -2113,32 +2176,7 @@ int fqsb_trigger(fqsb_system* s, uint64_t p, double eps, int direction)
 
 int fqsb_advance_to_fixed_force(fqsb_system* s, double f_frame, int allow_plastic)
 {
-    return guard([&] {
-        Group& g = *s->g;
-        const size_t N = (size_t)g.N;
-        std::vector<int64_t> st(N), st2(N);
-        std::vector<int32_t> jl(N), jl2(N);
-        std::vector<double> ff(N);
-        g.d2h(st.data(), g.start, N);
-        g.d2h(jl.data(), g.jloc, N);
-        g.d2h(ff.data(), g.f_frame, N);
-        double mean = 0.0;
-        for (double x : ff) {
-            mean += x;
-        }
-        mean /= (double)N;
-        std::vector<double> du(1, (f_frame - mean) / g.d.mu);
-        g.advance_uniformly(du, nullptr);
-        g.d2h(st2.data(), g.start, N);
-        g.d2h(jl2.data(), g.jloc, N);
-        bool same = true;
-        for (size_t p = 0; p < N; ++p) {
-            same &= st[p] + jl[p] == st2[p] + jl2[p];
-        }
-        if (!allow_plastic && !same) {
-            throw Error("assertion failed (allow_plastic || xt::all(xt::equal(m_chunk->index_at_align(), i_n)))");
-        }
-    });
+    return guard([&] { s->g->advance_to_fixed_force(&f_frame, allow_plastic != 0); });
 }
 
 uint64_t fqsb_chunk_size(const fqsb_system* s)
@@ -2410,6 +2448,25 @@ int fqsb_batch_quasistatic_stats(const fqsb_batch* b, float* ms, int64_t* launch
         *ms = b->g->qs_ms;
         *launches = b->g->qs_launches;
     });
+}
+
+int fqsb_batch_minimise_truncate(fqsb_batch* b, const int64_t* i_n, uint64_t A_truncate, uint64_t S_truncate,
+                                 double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                                 uint64_t* ret)
+{
+    return guard([&] {
+        b->g->minimise_truncate(i_n, A_truncate, S_truncate, tol, niter_tol, max_iter, max_iter_is_error != 0, ret);
+    });
+}
+
+int fqsb_batch_trigger(fqsb_batch* b, const uint64_t* p, double eps, int direction)
+{
+    return guard([&] { b->g->trigger_each(p, eps, direction); });
+}
+
+int fqsb_batch_advance_to_fixed_force(fqsb_batch* b, const double* f_frame, int allow_plastic)
+{
+    return guard([&] { b->g->advance_to_fixed_force(f_frame, allow_plastic != 0); });
 }
 
 int fqsb_batch_get(fqsb_batch* b, int field, double* out)

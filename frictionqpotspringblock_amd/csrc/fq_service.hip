@@ -26,6 +26,13 @@ hipError_t launch_trigger(const SysView* views, int64_t p, double eps, int32_t d
     return hipGetLastError();
 }
 
+hipError_t launch_trigger_each(const SysView* views, int nsys, const int64_t* ps, double eps, int32_t direction,
+                               int32_t do_align, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_trigger_each, dim3(nsys), dim3(1), 0, stream, views, ps, eps, direction, do_align);
+    return hipGetLastError();
+}
+
 hipError_t launch_refill(const SysView* views, int nsys, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_refill, dim3(nsys), dim3(SVC_THREADS), 0, stream, views);

@@ -221,11 +221,10 @@ __global__ void __launch_bounds__(SVC_THREADS) k_max_uniform(const SysView* __re
     }
 }
 
-// trigger: move particle p just past the edge of its well (forces untouched).
-__global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double eps, int32_t direction,
-                          int32_t do_align)
+// trigger: move particle p just past the edge of its well (forces untouched,
+// detail.h:1972-1977; Cuspy aligns first, detail.h:193-205).
+__device__ void trigger_one(const SysView& S, int64_t p, double eps, int32_t direction, int32_t do_align)
 {
-    const SysView S = views[blockIdx.x];
     const Landscape& L = S.ls;
     int32_t j = L.jloc[p];
     double yl = L.y[p * L.nchunk + j], yr = L.y[p * L.nchunk + j + 1];
@@ -233,6 +232,23 @@ __global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double e
         align_particle(L, p, S.u[p], yl, yr, j);
     }
     S.u[p] = direction > 0 ? yr + 0.5 * eps : yl - 0.5 * eps;
+}
+
+__global__ void k_trigger(const SysView* __restrict__ views, int64_t p, double eps, int32_t direction,
+                          int32_t do_align)
+{
+    trigger_one(views[blockIdx.x], p, eps, direction, do_align);
+}
+
+// trigger of particle ps[r] in every system r (one thread per system; a
+// negative index leaves the system as it is).
+__global__ void k_trigger_each(const SysView* __restrict__ views, const int64_t* __restrict__ ps, double eps,
+                               int32_t direction, int32_t do_align)
+{
+    const int64_t p = ps[blockIdx.x];
+    if (p >= 0) {
+        trigger_one(views[blockIdx.x], p, eps, direction, do_align);
+    }
 }
 
 // Landscape refill requested by a launch (status ST_REFILL): align every

@@ -399,6 +399,75 @@ PYBIND11_MODULE(_fqsb, m)
         }
     }
 
+    // FrictionQPotSpringBlock.Particles (python/main.cpp:278-470)
+    {
+        py::module sm = m.def_submodule("Particles", "Particles");
+        namespace SM = M::Particles;
+        sm.def("version_dependencies", &SM::version_dependencies,
+               "Return version information of library and its dependencies.");
+        sm.def("version_compiler", &SM::version_compiler, "Return compiler information.");
+        using Sh = SM::Shape;
+        using Par = std::vector<double>;
+        {
+            using S = SM::System_Cuspy;
+            py::class_<S, M::detail::System> cls(sm, "System_Cuspy");
+            cls.def(py::init<double, double, double, double, double, const Sh&, uint64_t, const std::string&,
+                             const Par&, double, size_t>(),
+                    "Constructor.", py::arg("m"), py::arg("eta"), py::arg("mu"), py::arg("k_frame"), py::arg("dt"),
+                    py::arg("shape"), py::arg("seed"), py::arg("distribution"), py::arg("parameters"),
+                    py::arg("offset") = -100.0, py::arg("nchunk") = 5000);
+            athermal(cls);
+            dynamics(cls);
+            // the reference's text (python/main.cpp:338)
+            cls.def("__repr__", [](const S&) { return "<FrictionQPotSpringBlock.Particle.System_Cuspy>"; });
+        }
+        {
+            using S = SM::System_Cuspy_RandomForcing;
+            py::class_<S, M::detail::System> cls(sm, "System_Cuspy_RandomForcing");
+            cls.def(py::init([](double m_, double eta, double mu, double k_frame, double dt, double mean,
+                                double stddev, uint64_t seed_forcing, const py::object& dinc_init,
+                                const py::object& dinc, const Sh& shape, uint64_t seed,
+                                const std::string& distribution, const Par& parameters, double offset,
+                                size_t nchunk) {
+                        return new S(m_, eta, mu, k_frame, dt, mean, stddev, seed_forcing,
+                                     as_i64(dinc_init, shape[0]), as_i64(dinc, shape[0]), shape, seed,
+                                     distribution, parameters, offset, nchunk);
+                    }),
+                    "Constructor.", py::arg("m"), py::arg("eta"), py::arg("mu"), py::arg("k_frame"), py::arg("dt"),
+                    py::arg("mean"), py::arg("stddev"), py::arg("seed_forcing"), py::arg("dinc_init"),
+                    py::arg("dinc"), py::arg("shape"), py::arg("seed"), py::arg("distribution"),
+                    py::arg("parameters"), py::arg("offset") = -100.0, py::arg("nchunk") = 5000);
+            dynamics(cls);
+            external(cls);
+            cls.def("__repr__",
+                    [](const S&) { return "<FrictionQPotSpringBlock.Particles.System_Cuspy_RandomForcing>"; });
+        }
+        {
+            using S = SM::System_SemiSmooth;
+            py::class_<S, M::detail::System> cls(sm, "System_SemiSmooth");
+            cls.def(py::init<double, double, double, double, double, double, const Sh&, uint64_t,
+                             const std::string&, const Par&, double, size_t>(),
+                    "Constructor.", py::arg("m"), py::arg("eta"), py::arg("mu"), py::arg("kappa"),
+                    py::arg("k_frame"), py::arg("dt"), py::arg("shape"), py::arg("seed"), py::arg("distribution"),
+                    py::arg("parameters"), py::arg("offset") = -100.0, py::arg("nchunk") = 5000);
+            athermal(cls);
+            dynamics(cls);
+            cls.def("__repr__", [](const S&) { return "<FrictionQPotSpringBlock.Particles.System_SemiSmooth>"; });
+        }
+        {
+            using S = SM::System_Smooth;
+            py::class_<S, M::detail::System> cls(sm, "System_Smooth");
+            cls.def(py::init<double, double, double, double, double, const Sh&, uint64_t, const std::string&,
+                             const Par&, double, size_t>(),
+                    "Constructor.", py::arg("m"), py::arg("eta"), py::arg("mu"), py::arg("k_frame"), py::arg("dt"),
+                    py::arg("shape"), py::arg("seed"), py::arg("distribution"), py::arg("parameters"),
+                    py::arg("offset") = -100.0, py::arg("nchunk") = 5000);
+            athermal(cls);
+            dynamics(cls);
+            cls.def("__repr__", [](const S&) { return "<FrictionQPotSpringBlock.Particles.System_Smooth>"; });
+        }
+    }
+
     // FrictionQPotSpringBlock.Line2d
     {
         py::module sm = m.def_submodule("Line2d", "Line2d");

@@ -32,6 +32,9 @@ _CLASSES = {
     "System_Smooth_Quartic": dict(potential="Smooth", interactions="Quartic1d"),
     "System_Cuspy_QuarticGradient": dict(potential="Cuspy", interactions="QuarticGradient1d"),
     "System_Cuspy_LongRange": dict(potential="Cuspy", interactions="LongRange1d"),
+    "Particles.System_Cuspy": dict(potential="Cuspy", interactions=None),
+    "Particles.System_SemiSmooth": dict(potential="SemiSmooth", interactions=None),
+    "Particles.System_Smooth": dict(potential="Smooth", interactions=None),
     "Line2d.System_Cuspy_Laplace": dict(potential="Cuspy", interactions="Laplace2d"),
     "Line2d.System_Cuspy_QuarticGradient": dict(potential="Cuspy", interactions="QuarticGradient2d"),
 }

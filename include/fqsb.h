@@ -264,6 +264,16 @@ int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, 
  * stream around its device work (ms), and the number of driver launches (0
  * if it was driven event by event from the host). */
 int fqsb_batch_quasistatic_stats(const fqsb_batch* b, float* ms, int64_t* launches);
+/* minimise_truncate (detail.h:1833-1893) on all replicas: i_n [R, N]; ret[r]. */
+int fqsb_batch_minimise_truncate(fqsb_batch* b, const int64_t* i_n, uint64_t A_truncate, uint64_t S_truncate,
+                                 double tol, uint64_t niter_tol, uint64_t max_iter, int max_iter_is_error,
+                                 uint64_t* ret);
+/* trigger (detail.h:1972-1977) of particle p[r] in replica r; forces are not
+ * recomputed, as in the reference. */
+int fqsb_batch_trigger(fqsb_batch* b, const uint64_t* p, double eps, int direction);
+/* advanceToFixedForce (detail.h:1988-1995) with target mean frame force
+ * f_frame[r] for replica r. */
+int fqsb_batch_advance_to_fixed_force(fqsb_batch* b, const double* f_frame, int allow_plastic);
 /* Arrays of all replicas, [R, N]; scalars [R]. */
 int fqsb_batch_get(fqsb_batch* b, int field, double* out);
 int fqsb_batch_set(fqsb_batch* b, int field, const double* in);

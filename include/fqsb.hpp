@@ -686,6 +686,97 @@ public:
 
 } // namespace Line1d
 
+namespace Particles {
+
+using Shape = std::array<size_t, 1>;
+
+inline std::vector<std::string> version_dependencies()
+{
+    return Line1d::version_dependencies();
+}
+
+inline std::vector<std::string> version_compiler()
+{
+    return Line1d::version_compiler();
+}
+
+// Independent particles, each in its own landscape (no interactions).
+#define FQSB_PARTICLES_DESC                                                                                         \
+    detail::make_desc(1, {shape[0]}, seed, distribution, parameters, offset, nchunk)
+
+/* Particles.h:93-151 */
+class System_Cuspy : public detail::System {
+public:
+    System_Cuspy(double m, double eta, double mu, double k_frame, double dt, const Shape& shape, uint64_t seed,
+                 const std::string& distribution, const std::vector<double>& parameters, double offset = -100.0,
+                 size_t nchunk = 5000)
+    {
+        fqsb_desc d = detail::physics(FQSB_PARTICLES_DESC, m, eta, mu, k_frame, dt);
+        d.potential = FQSB_POT_CUSPY;
+        d.interactions = FQSB_INT_NONE;
+        init(d, {shape[0]});
+    }
+};
+
+/* Particles.h:161-231 */
+class System_Cuspy_RandomForcing : public detail::System {
+public:
+    System_Cuspy_RandomForcing(double m, double eta, double mu, double k_frame, double dt, double mean,
+                               double stddev, uint64_t seed_forcing, const std::vector<int64_t>& dinc_init,
+                               const std::vector<int64_t>& dinc, const Shape& shape, uint64_t seed,
+                               const std::string& distribution, const std::vector<double>& parameters,
+                               double offset = -100.0, size_t nchunk = 5000)
+    {
+        fqsb_desc d = detail::physics(FQSB_PARTICLES_DESC, m, eta, mu, k_frame, dt);
+        d.potential = FQSB_POT_CUSPY;
+        d.interactions = FQSB_INT_NONE;
+        d.external = FQSB_EXT_RANDOMFORCING;
+        d.minimisation = FQSB_MIN_NONE;
+        d.mean = mean;
+        d.stddev = stddev;
+        d.seed_forcing = seed_forcing;
+        if (dinc_init.size() != shape[0] || dinc.size() != shape[0]) {
+            throw std::runtime_error("dinc_init and dinc must have the shape of the system");
+        }
+        d.dinc_init = dinc_init.data();
+        d.dinc = dinc.data();
+        init(d, {shape[0]});
+    }
+};
+
+/* Particles.h:233-274 */
+class System_SemiSmooth : public detail::System {
+public:
+    System_SemiSmooth(double m, double eta, double mu, double kappa, double k_frame, double dt, const Shape& shape,
+                      uint64_t seed, const std::string& distribution, const std::vector<double>& parameters,
+                      double offset = -100.0, size_t nchunk = 5000)
+    {
+        fqsb_desc d = detail::physics(FQSB_PARTICLES_DESC, m, eta, mu, k_frame, dt);
+        d.potential = FQSB_POT_SEMISMOOTH;
+        d.interactions = FQSB_INT_NONE;
+        d.kappa = kappa;
+        init(d, {shape[0]});
+    }
+};
+
+/* Particles.h:276-311 */
+class System_Smooth : public detail::System {
+public:
+    System_Smooth(double m, double eta, double mu, double k_frame, double dt, const Shape& shape, uint64_t seed,
+                  const std::string& distribution, const std::vector<double>& parameters, double offset = -100.0,
+                  size_t nchunk = 5000)
+    {
+        fqsb_desc d = detail::physics(FQSB_PARTICLES_DESC, m, eta, mu, k_frame, dt);
+        d.potential = FQSB_POT_SMOOTH;
+        d.interactions = FQSB_INT_NONE;
+        init(d, {shape[0]});
+    }
+};
+
+#undef FQSB_PARTICLES_DESC
+
+} // namespace Particles
+
 namespace Line2d {
 
 using Shape = std::array<size_t, 2>;

@@ -585,6 +585,13 @@ Line1d = types.SimpleNamespace(
     System_Cuspy_LongRange=_l1("Cuspy", "LongRange1d")("System_Cuspy_LongRange"),
 )
 
+Particles = types.SimpleNamespace(
+    System_Cuspy=_l1("Cuspy", None)("System_Cuspy"),
+    System_Cuspy_RandomForcing=_l1("Cuspy", None, external=1, minimisation=2)("System_Cuspy_RandomForcing"),
+    System_SemiSmooth=_l1("SemiSmooth", None)("System_SemiSmooth"),
+    System_Smooth=_l1("Smooth", None)("System_Smooth"),
+)
+
 Line2d = types.SimpleNamespace(
     System_Cuspy_Laplace=_l2("Cuspy", "Laplace2d")("System_Cuspy_Laplace"),
     System_Cuspy_QuarticGradient=_l2("Cuspy", "QuarticGradient2d")("System_Cuspy_QuarticGradient"),

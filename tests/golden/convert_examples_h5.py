@@ -25,6 +25,7 @@ NAMES = [
     "Line1d_SemiSmooth_Laplace",
     "Line1d_System_Cuspy_Laplace_RandomForcing",
     "Line2d_Cuspy_Laplace",
+    "Particles_Cuspy",
 ]
 
 

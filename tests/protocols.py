@@ -59,11 +59,18 @@ EXAMPLES = {
     ),
     "Line1d_Cuspy_Laplace_Nopassing": ("Line1d", "System_Cuspy_Laplace_Nopassing", _nopassing()),
     "Line2d_Cuspy_Laplace": ("Line2d", "System_Cuspy_Laplace", _line2d()),
+    "Particles_Cuspy": ("Particles", "System_Cuspy", dict(_COMMON)),
 }
 
 
 def golden(name):
-    return np.load(os.path.join(GOLDEN, name + ".npz"))
+    """The fixture's arrays; the frame position is "x_frame" (the Particles
+    example stores it as "u_frame")."""
+    with np.load(os.path.join(GOLDEN, name + ".npz")) as f:
+        ref = {k: f[k] for k in f.files}
+    if "x_frame" not in ref and "u_frame" in ref:
+        ref["x_frame"] = ref["u_frame"]
+    return ref
 
 
 def example_system(module, name):

@@ -82,6 +82,9 @@ SYSTEMS = [
     ("Line1d", "System_Cuspy_LongRange", 300, dict(k_interactions=1.0, alpha=1.5)),
     ("Line2d", "System_Cuspy_Laplace", (30, 40), dict(k_interactions=1.0)),
     ("Line2d", "System_Cuspy_QuarticGradient", (20, 25), dict(k2=1.0, k4=1.0)),
+    ("Particles", "System_Cuspy", 700, {}),
+    ("Particles", "System_SemiSmooth", 700, dict(kappa=1.0)),
+    ("Particles", "System_Smooth", 700, {}),
 ]
 
 
@@ -120,7 +123,7 @@ def test_dynamics_and_quasistatics(fq, oracle, path, sub, cls, n, extra):
     rb = b.minimise()
     assert ra == rb == 0
     assert_same_state(a, b, "minimise")
-    if cls != "System_Smooth_Laplace":
+    if not cls.startswith("System_Smooth"):  # Smooth: maxUniformDisplacement raises
         for kick in (False, True):
             da = a.eventDrivenStep(1e-3, kick)
             db = b.eventDrivenStep(1e-3, kick)
@@ -200,6 +203,31 @@ def test_random_forcing(fq, oracle, path, cls):
     assert not hasattr(a, "minimise")
 
 
+def test_particles_random_forcing(fq, oracle, path):
+    N = 600
+    rng = np.random.default_rng(3)
+    kw = dict(mean=0.01, stddev=0.1, seed_forcing=5, dinc_init=rng.integers(0, 12, N), dinc=rng.integers(1, 12, N))
+    a = _make(fq, "Particles", "System_Cuspy_RandomForcing", N, kw)
+    b = _make(oracle, "Particles", "System_Cuspy_RandomForcing", N, kw)
+    assert_same_state(a, b, "construction")
+    for s in (a, b):
+        s.flowSteps(250, 0.1)
+    assert_same_state(a, b, "flowSteps with random forcing")
+    assert np.array_equal(a.external.f_thermal, b.external.f_thermal)
+    assert not hasattr(a, "minimise") and not hasattr(a, "eventDrivenStep")
+
+
+def test_particles_example_driver(fq):
+    """examples/Particles_Cuspy.py in full (2000 events) through the
+    quasistatic driver against the reference's stored output."""
+    ref = P.golden("Particles_Cuspy")
+    a = P.example_system(fq, "Particles_Cuspy")
+    q = a.quasistatic(1e-3, len(ref["S"]))
+    assert np.array_equal(q["S"], ref["S"])
+    assert np.allclose(q["u_frame"], ref["x_frame"])
+    assert np.allclose(q["f_frame"], ref["f_frame"])
+
+
 def test_reference_random_forcing(fq, oracle):
     RC.random_forcing_draws(fq, oracle)
 
@@ -218,6 +246,7 @@ def test_reference_case(fq, case):
         ("Line1d_Cuspy_Laplace_Nopassing", 400),
         ("Line2d_Cuspy_Laplace", 60),
         ("Line1d_Cuspy_Laplace_LongRange", 6),
+        ("Particles_Cuspy", 400),
     ],
 )
 def test_golden_examples(fq, oracle, path, name, nstep):

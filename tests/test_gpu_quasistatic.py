@@ -45,6 +45,8 @@ SYSTEMS = [
     ("Line1d", "System_Cuspy_LongRange", 300, dict(k_interactions=1.0, alpha=1.5)),
     ("Line2d", "System_Cuspy_Laplace", (30, 40), dict(k_interactions=1.0)),
     ("Line2d", "System_Cuspy_QuarticGradient", (20, 25), dict(k2=1.0, k4=1.0)),
+    ("Particles", "System_Cuspy", 900, {}),
+    ("Particles", "System_SemiSmooth", 900, dict(kappa=1.0)),
 ]
 
 
@@ -234,3 +236,63 @@ def test_errors(fq):
     # systems with random forcing have no minimise, hence no driver
     assert not hasattr(fq.Line1d.System_Cuspy_Laplace_RandomForcing, "quasistatic")
     assert a.quasistatic(1e-3, 0)["S"].shape == (0,)
+
+
+def test_batch_truncate_trigger_fixed_force(fq, oracle):
+    """Batched minimise_truncate, trigger and advanceToFixedForce: replica r
+    equals an oracle system with seed 3 + r doing the same operation with its
+    own arguments."""
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R, N = 3, 400
+    kw = dict(
+        potential="Cuspy",
+        interactions="Laplace1d",
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=3,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=60,
+    )
+    b = Batch(R, seed_stride=1, **kw)
+    o = [oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, seed=3 + r)) for r in range(R)]
+
+    def same(what):
+        for r in range(R):
+            assert np.array_equal(b.u[r], o[r].u), (what, r)
+            assert np.array_equal(b.f[r], o[r].f), (what, r)
+            assert np.array_equal(b.index_at_align[r], o[r].chunk.index_at_align), (what, r)
+            assert b.inc[r] == o[r].inc and b.u_frame[r] == o[r].u_frame, (what, r)
+
+    b.quasistatic(1e-3, 1)
+    for s in o:
+        s.u_frame = 0.0
+        assert s.minimise() == 0
+    same("quench")
+    # advance to fixed mean frame forces (elastic)
+    target = [float(np.mean(s.f_frame)) + 1e-4 * (r + 1) for r, s in enumerate(o)]
+    b.advanceToFixedForce(target)
+    for r, s in enumerate(o):
+        s.advanceToFixedForce(target[r])
+    same("advanceToFixedForce")
+    # trigger a different particle in each replica, then a truncated minimise
+    p = [5, 17, 123]
+    i_n = b.index_at_align.copy()
+    b.trigger(p, 1e-3)
+    for r, s in enumerate(o):
+        s.trigger(p[r], 1e-3)
+    for r in range(R):
+        assert np.array_equal(b.u[r], o[r].u)
+    ret = b.minimise_truncate(i_n, A_truncate=3)
+    for r, s in enumerate(o):
+        assert ret[r] == s.minimise_truncate(i_n[r], A_truncate=3), r
+    same("minimise_truncate")
+    with pytest.raises(RuntimeError, match="out of range"):
+        b.trigger([0, N, 0], 1e-3)

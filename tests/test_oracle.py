@@ -26,6 +26,7 @@ PREFIX = {
     "Line1d_Cuspy_Laplace_LongRange": 16,
     "Line1d_Cuspy_Laplace_Nopassing": 1000,
     "Line2d_Cuspy_Laplace": 100,
+    "Particles_Cuspy": 400,
 }
 
 
