@@ -43,6 +43,21 @@ DISTRIBUTION = {
 EXTERNAL = {None: 0, "RandomNormalForcing": 1}
 MINIMISATION = {"Inertial": 0, "Overdamped": 1, "None": 2}
 LONGRANGE_METHOD = {"exact": 0, "matrix": 1}
+# checkpoint items (include/fqsb.h FQSB_STATE_*) -> (id, dtype)
+STATE = {
+    "y": (0, np.float64),
+    "start": (1, np.int64),
+    "jloc": (2, np.int32),
+    "st_begin": (3, np.uint64),
+    "st_end": (4, np.uint64),
+    "f_thermal": (5, np.float64),
+    "next": (6, np.int64),
+    "inc": (7, np.int64),
+    "u_frame": (8, np.float64),
+    "rng_state": (9, np.uint64),
+    "qs_first": (10, np.int64),
+    "qs_last": (11, np.int64),
+}
 FIELD = {
     "u": 0,
     "v": 1,
@@ -168,6 +183,10 @@ SIGNATURES = {
         [_P, _ip, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _up],
     ),
     "fqsb_batch_trigger": (C.c_int, [_P, _up, C.c_double, C.c_int]),
+    "fqsb_batch_state_size": (C.c_int, [_P, C.c_int, _up, C.POINTER(C.c_int)]),
+    "fqsb_batch_get_state": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "fqsb_batch_set_state": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "fqsb_batch_refresh": (C.c_int, [_P]),
     "fqsb_batch_advance_to_fixed_force": (C.c_int, [_P, _dp, C.c_int]),
     "fqsb_quasistatic": (
         C.c_int,
@@ -260,6 +279,14 @@ def _quasistatic(call, nsys, eps, nevents, first_event, direction, kicks_only, t
     ]
     _check(call(*args))
     return out
+
+
+def save_events(path, q, system=None):
+    """Per-event records of `quasistatic` in the examples' output schema
+    (examples/Line1d_Cuspy_Laplace.py:63-67: "x_frame", "f_frame", "S"; plus
+    "inc"), as an .npz file; `system` picks one row of a batch's records."""
+    pick = (lambda x: x) if system is None else (lambda x: np.asarray(x)[system])
+    np.savez(path, x_frame=pick(q["u_frame"]), f_frame=pick(q["f_frame"]), S=pick(q["S"]), inc=pick(q["inc"]))
 
 
 def _check(rc):
@@ -520,6 +547,58 @@ class Batch:
         n = C.c_int64()
         _check(capi().fqsb_batch_quasistatic_stats(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # checkpoint / restart ---------------------------------------------------
+
+    def _get_state(self, name):
+        what, dtype = STATE[name]
+        n, b = C.c_uint64(), C.c_int()
+        _check(capi().fqsb_batch_state_size(self._h, what, C.byref(n), C.byref(b)))
+        out = np.empty(n.value, dtype=dtype)
+        assert out.itemsize == b.value
+        _check(capi().fqsb_batch_get_state(self._h, what, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def _set_state(self, name, value):
+        what, dtype = STATE[name]
+        n, b = C.c_uint64(), C.c_int()
+        _check(capi().fqsb_batch_state_size(self._h, what, C.byref(n), C.byref(b)))
+        v = np.ascontiguousarray(value, dtype=dtype).ravel()
+        if v.size != n.value:
+            raise RuntimeError(f"checkpoint item {name} has {v.size} values, the batch needs {n.value}")
+        _check(capi().fqsb_batch_set_state(self._h, what, v.ctypes.data_as(C.c_void_p)))
+
+    def checkpoint(self):
+        """Everything that fixes the continuation bit for bit: u, v, a, the
+        landscape windows with their pcg32 states, inc, u_frame, the activity
+        counters and, with random forcing, its draws and state (the batched
+        form of the reference's restart, test_Line1d.py:309-341). A dict of
+        numpy arrays; `save` writes it as an .npz file."""
+        ck = {k: self._get(k) for k in ("u", "v", "a")}
+        items = ["y", "start", "jloc", "st_begin", "st_end", "inc", "u_frame", "qs_first", "qs_last"]
+        if self._desc.external:
+            items += ["f_thermal", "next", "rng_state"]
+        for k in items:
+            ck[k] = self._get_state(k)
+        return ck
+
+    def restore(self, ck):
+        """Continue from a checkpoint of a batch of the same description."""
+        for k in ("y", "start", "jloc", "st_begin", "st_end", "inc", "u_frame", "qs_first", "qs_last"):
+            self._set_state(k, ck[k])
+        if self._desc.external:
+            for k in ("f_thermal", "next", "rng_state"):
+                self._set_state(k, ck[k])
+        for k in ("u", "v", "a"):
+            self._set(k, ck[k])
+        _check(capi().fqsb_batch_refresh(self._h))
+
+    def save(self, path):
+        np.savez(path, **self.checkpoint())
+
+    def load(self, path):
+        with np.load(path) as f:
+            self.restore({k: f[k] for k in f.files})
 
     def _get(self, name):
         out = np.empty(self._R * self._N)

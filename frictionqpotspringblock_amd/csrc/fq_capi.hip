@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1799,6 +1800,53 @@ std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64
     return conv;
 }
 
+// Batch state for checkpoints: where each FQSB_STATE_* item lives.
+struct StateItem {
+    void* dev;       // device array, or null for a per-system scalar in RunState
+    size_t count;    // elements
+    int bytes;       // bytes per element
+    size_t rs_off;   // offset of the scalar in RunState (dev == null)
+};
+
+StateItem state_item(Group& g, int what)
+{
+    const size_t T = (size_t)g.total, R = (size_t)g.R;
+    auto thermal = [&](void* p) {
+        if (!g.ext) {
+            throw Error("system has no external forcing");
+        }
+        return p;
+    };
+    switch (what) {
+        case FQSB_STATE_Y:
+            return {g.y, T * (size_t)g.nchunk, 8, 0};
+        case FQSB_STATE_START:
+            return {g.start, T, 8, 0};
+        case FQSB_STATE_JLOC:
+            return {g.jloc, T, 4, 0};
+        case FQSB_STATE_ST_BEGIN:
+            return {g.st_begin, T, 8, 0};
+        case FQSB_STATE_ST_END:
+            return {g.st_end, T, 8, 0};
+        case FQSB_STATE_F_THERMAL:
+            return {thermal(g.f_th), T, 8, 0};
+        case FQSB_STATE_NEXT:
+            return {thermal(g.next), T, 8, 0};
+        case FQSB_STATE_INC:
+            return {nullptr, R, 8, offsetof(fq::RunState, inc)};
+        case FQSB_STATE_U_FRAME:
+            return {nullptr, R, 8, offsetof(fq::RunState, u_frame)};
+        case FQSB_STATE_RNG:
+            return {nullptr, R, 8, offsetof(fq::RunState, rng_state)};
+        case FQSB_STATE_QS_FIRST:
+            return {nullptr, R, 8, offsetof(fq::RunState, qs_first)};
+        case FQSB_STATE_QS_LAST:
+            return {nullptr, R, 8, offsetof(fq::RunState, qs_last)};
+        default:
+            throw Error("unknown state item");
+    }
+}
+
 } // namespace
 
 struct fqsb_system {
@@ -2467,6 +2515,64 @@ int fqsb_batch_trigger(fqsb_batch* b, const uint64_t* p, double eps, int directi
 int fqsb_batch_advance_to_fixed_force(fqsb_batch* b, const double* f_frame, int allow_plastic)
 {
     return guard([&] { b->g->advance_to_fixed_force(f_frame, allow_plastic != 0); });
+}
+
+
+int fqsb_batch_state_size(fqsb_batch* b, int what, uint64_t* count, int* bytes)
+{
+    return guard([&] {
+        const StateItem it = state_item(*b->g, what);
+        *count = (uint64_t)it.count;
+        *bytes = it.bytes;
+    });
+}
+
+int fqsb_batch_get_state(fqsb_batch* b, int what, void* out)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        const StateItem it = state_item(g, what);
+        if (it.dev) {
+            g.d2h((char*)out, (const char*)it.dev, it.count * (size_t)it.bytes);
+            return;
+        }
+        for (size_t r = 0; r < it.count; ++r) {
+            std::memcpy((char*)out + r * 8, (const char*)&g.hrs[r] + it.rs_off, 8);
+        }
+    });
+}
+
+int fqsb_batch_set_state(fqsb_batch* b, int what, const void* in)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        const StateItem it = state_item(g, what);
+        if (it.dev) {
+            g.h2d((char*)it.dev, (const char*)in, it.count * (size_t)it.bytes);
+        }
+        else {
+            for (size_t r = 0; r < it.count; ++r) {
+                std::memcpy((char*)&g.hrs[r] + it.rs_off, (const char*)in + r * 8, 8);
+            }
+        }
+        if (what == FQSB_STATE_NEXT) { // the kernels' cached minimum of next
+            const int64_t* nx = (const int64_t*)in;
+            for (int64_t r = 0; r < g.R; ++r) {
+                int64_t mn = std::numeric_limits<int64_t>::max();
+                for (int64_t p = g.off[(size_t)r]; p < g.off[(size_t)r + 1]; ++p) {
+                    mn = std::min(mn, nx[p]);
+                }
+                g.hrs[(size_t)r].min_next = mn;
+            }
+        }
+        g.push();
+        HIPC(hipStreamSynchronize(g.stream));
+    });
+}
+
+int fqsb_batch_refresh(fqsb_batch* b)
+{
+    return guard([&] { b->g->refresh(); });
 }
 
 int fqsb_batch_get(fqsb_batch* b, int field, double* out)

@@ -274,6 +274,32 @@ int fqsb_batch_trigger(fqsb_batch* b, const uint64_t* p, double eps, int directi
 /* advanceToFixedForce (detail.h:1988-1995) with target mean frame force
  * f_frame[r] for replica r. */
 int fqsb_batch_advance_to_fixed_force(fqsb_batch* b, const double* f_frame, int allow_plastic);
+/* Checkpoint / restart of a batch (the batched form of the reference's
+ * restart through chunk.restore and the state setters, test_Line1d.py:309-341):
+ * every item that, with u, v and a (fqsb_batch_get / fqsb_batch_set), fixes
+ * the continuation bit for bit. Per particle [R, N] (the landscape
+ * [R, N, nchunk]) or per replica [R]; element sizes from
+ * fqsb_batch_state_size. Setting an item does not recompute forces; call
+ * fqsb_batch_refresh after the last one. */
+enum {
+    FQSB_STATE_Y = 0,         /* f64 [R, N, nchunk]: the landscape windows */
+    FQSB_STATE_START = 1,     /* i64 [R, N]: global index of each window's first value */
+    FQSB_STATE_JLOC = 2,      /* i32 [R, N]: local index of the current well */
+    FQSB_STATE_ST_BEGIN = 3,  /* u64 [R, N]: pcg32 state after each window's first value */
+    FQSB_STATE_ST_END = 4,    /* u64 [R, N]: pcg32 state after each window's last value */
+    FQSB_STATE_F_THERMAL = 5, /* f64 [R, N]: random forcing (systems with external forcing) */
+    FQSB_STATE_NEXT = 6,      /* i64 [R, N]: increment of each particle's next draw */
+    FQSB_STATE_INC = 7,       /* i64 [R] */
+    FQSB_STATE_U_FRAME = 8,   /* f64 [R] */
+    FQSB_STATE_RNG = 9,       /* u64 [R]: pcg32 state of the random forcing */
+    FQSB_STATE_QS_FIRST = 10, /* i64 [R]: quasistaticActivityFirst */
+    FQSB_STATE_QS_LAST = 11   /* i64 [R]: quasistaticActivityLast */
+};
+int fqsb_batch_state_size(fqsb_batch* b, int what, uint64_t* count, int* bytes);
+int fqsb_batch_get_state(fqsb_batch* b, int what, void* out);
+int fqsb_batch_set_state(fqsb_batch* b, int what, const void* in);
+/* refresh (detail.h:1310-1315) of every replica: all forces from the state. */
+int fqsb_batch_refresh(fqsb_batch* b);
 /* Arrays of all replicas, [R, N]; scalars [R]. */
 int fqsb_batch_get(fqsb_batch* b, int field, double* out);
 int fqsb_batch_set(fqsb_batch* b, int field, const double* in);

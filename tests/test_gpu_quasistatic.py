@@ -296,3 +296,79 @@ def test_batch_truncate_trigger_fixed_force(fq, oracle):
     same("minimise_truncate")
     with pytest.raises(RuntimeError, match="out of range"):
         b.trigger([0, N, 0], 1e-3)
+
+
+def _rf_kw(N):
+    rng = np.random.default_rng(11)
+    return dict(
+        external="RandomNormalForcing",
+        minimisation="None",
+        mean=0.01,
+        stddev=0.1,
+        seed_forcing=5,
+        dinc_init=rng.integers(0, 12, N),
+        dinc=rng.integers(1, 12, N),
+    )
+
+
+@pytest.mark.parametrize("forcing", [False, True])
+def test_batch_checkpoint_restart(fq, tmp_path, forcing):
+    """A batch restored from a checkpoint (in place, and into a fresh batch
+    from the .npz file) continues bit for bit: the same events, states and
+    landscape windows (refills included: 40-well windows)."""
+    from frictionqpotspringblock_amd.batch import Batch, save_events
+
+    R, N = 3, 300
+    kw = dict(
+        potential="Cuspy",
+        interactions="Laplace1d",
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=3,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=40,
+    )
+    if forcing:
+        kw.update(_rf_kw(N))
+
+    def advance(b):
+        if forcing:
+            b.flowSteps(700, 0.05)
+            return None
+        return b.quasistatic(1e-3, 16, first_event=11)
+
+    b = Batch(R, seed_stride=1, **kw)
+    if forcing:
+        b.flowSteps(500, 0.05)
+    else:
+        b.quasistatic(1e-3, 11)
+    ck = b.checkpoint()
+    b.save(tmp_path / "ck.npz")
+    qa = advance(b)
+    after = b.checkpoint()
+    b.restore(ck)
+    qb = advance(b)
+    fresh = Batch(R, seed_stride=1, **kw)
+    fresh.load(tmp_path / "ck.npz")
+    qc = advance(fresh)
+    for other, q in ((b, qb), (fresh, qc)):
+        got = other.checkpoint()
+        for k, v in after.items():
+            assert np.array_equal(got[k], v), k
+        assert np.array_equal(other.f, b.f)
+        if q is not None:
+            for k in qa:
+                assert np.array_equal(q[k], qa[k]), k
+    assert not forcing or not np.array_equal(after["next"], ck["next"])
+    assert forcing or np.any(after["start"] != ck["start"])  # windows moved
+    if qa is not None:
+        save_events(tmp_path / "events.npz", qa, system=1)
+        with np.load(tmp_path / "events.npz") as f:
+            assert np.array_equal(f["S"], qa["S"][1]) and np.array_equal(f["x_frame"], qa["u_frame"][1])
