@@ -144,6 +144,14 @@ struct Group {
     int32_t* tile_first = nullptr;
     double* tile_part = nullptr;
     int32_t* tile_flag = nullptr;
+    // blocks of tile_k steps per launch for chains (fq_tile.hpp k_tile_block;
+    // FQSB_TILE_K=0: one launch per step)
+    int tile_k = 0;
+    int tb_ntiles = 0;                // tiles of the blocked path (2048 particles)
+    fq::TileDesc* tb_tiles = nullptr;
+    int32_t* tb_first = nullptr;      // [R + 1]
+    double* tile_part_k = nullptr;    // [tb_ntiles][tile_k][2]
+    int32_t* tile_fail = nullptr;     // [tb_ntiles]
     // captured blocks of tiled steps (hipGraph), keyed by the run arguments and
     // the number of steps; FQSB_NO_GRAPHS=1 launches the kernels one by one
     struct TileGraph {
@@ -219,6 +227,10 @@ struct Group {
         dfree(tile_first);
         dfree(tile_part);
         dfree(tile_flag);
+        dfree(tile_part_k);
+        dfree(tile_fail);
+        dfree(tb_tiles);
+        dfree(tb_first);
         dfree(lr_yl);
         dfree(lr_yr);
         if (ev0) {
@@ -431,12 +443,15 @@ struct Group {
         for (auto& r : hrs) {
             r.target = (int64_t)total;
             r.cur = 0;
+            r.tb_redo = 0;
+            r.tb_fail = 0;
         }
         push();
         if (timed) {
             HIPC(hipEventRecord(ev0, stream));
         }
         HIPC(fq::launch_tile_prologue(views, tiles, ntiles, stream));
+        const bool blocks = tile_k > 0 && tile_part_k != nullptr && fq::tile_block_supported(pot, inter);
         auto normalise = [&] {
             if (!fq::launch_tile_finish(pot, inter, views, tiles, ntiles, stream)) {
                 throw Error("no tiled kernel for this potential / interaction");
@@ -449,6 +464,18 @@ struct Group {
             push();
         };
         auto enqueue = [&](uint64_t nb) {
+            if (blocks) { // blocks of tile_k steps (decide commits or redoes)
+                for (uint64_t s = 0; s < (nb + (uint64_t)tile_k - 1) / (uint64_t)tile_k; ++s) {
+                    if (!fq::launch_tile_block(pot, inter, views, tb_tiles, tb_ntiles, tile_part_k, tile_fail, ra,
+                                               tile_k, stream)) {
+                        throw Error("no blocked tile kernel for this potential / interaction");
+                    }
+                    HIPC(hipGetLastError());
+                    HIPC(fq::launch_tile_block_decide(views, (int)R, tb_first, tile_part_k, tile_fail, ra, tile_k,
+                                                      stream));
+                }
+                return;
+            }
             for (uint64_t s = 0; s < nb; ++s) {
                 if (!fq::launch_tile_step(pot, inter, views, tiles, ntiles, tile_lds, tile_part, tile_flag, ra,
                                           stream)) {
@@ -1611,6 +1638,32 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds)
             g->tile_first = dalloc<int32_t>((size_t)R + 1);
             g->h2d(g->tile_first, first.data(), (size_t)R + 1);
             g->tile_part = dalloc<double>(2 * tl.size());
+            g->tile_k = 32;
+            if (const char* e = std::getenv("FQSB_TILE_K")) {
+                g->tile_k = std::max(0, std::min(std::atoi(e), 256));
+            }
+            if (g->tile_k > 0 && fq::tile_block_supported(g->pot, inter)) {
+                std::vector<fq::TileDesc> tb;
+                std::vector<int32_t> tbf((size_t)R + 1, 0);
+                for (int64_t r = 0; r < R; ++r) {
+                    tbf[(size_t)r] = (int32_t)tb.size();
+                    for (int64_t p0 = 0; p0 < Ns[(size_t)r]; p0 += fq::TB_TILE_PARTICLES) {
+                        fq::TileDesc td{};
+                        td.sys = (int32_t)r;
+                        td.p0 = p0;
+                        td.p1 = std::min<int64_t>(Ns[(size_t)r], p0 + fq::TB_TILE_PARTICLES);
+                        tb.push_back(td);
+                    }
+                }
+                tbf[(size_t)R] = (int32_t)tb.size();
+                g->tb_ntiles = (int)tb.size();
+                g->tb_tiles = dalloc<fq::TileDesc>(tb.size());
+                g->h2d(g->tb_tiles, tb.data(), tb.size());
+                g->tb_first = dalloc<int32_t>((size_t)R + 1);
+                g->h2d(g->tb_first, tbf.data(), (size_t)R + 1);
+                g->tile_part_k = dalloc<double>(2 * (size_t)g->tile_k * tb.size());
+                g->tile_fail = dalloc<int32_t>(tb.size());
+            }
             g->tile_flag = dalloc<int32_t>((size_t)R);
         }
     }

@@ -128,8 +128,12 @@ struct RunState {
     // sum of (well index - i_ref) over the particles when the event began
     int64_t qs_event;
     int32_t qs_phase;
-    int32_t qs_pad;
+    // tiled blocks (k_tile_block): steps the next block must redo (0: a new
+    // block), and the failure (TILE_FLAG_*) to report after committing them
+    int32_t tb_redo;
     int64_t qs_isum;
+    int32_t tb_fail;
+    int32_t tb_pad;
 };
 
 // Device view of one system (all pointers are device pointers).
@@ -157,6 +161,9 @@ struct SysView {
 // Uniform arguments of one resident launch.
 // One tile of the tiled per-step path (fq_tile.hpp): particles [p0, p1) of
 // system `sys` (whole rows for lattices).
+// particles per tile of the blocked tiled path (k_tile_block)
+constexpr int64_t TB_TILE_PARTICLES = 2048;
+
 struct TileDesc {
     int32_t sys;
     int32_t pad;

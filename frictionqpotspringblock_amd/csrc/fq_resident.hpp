@@ -569,23 +569,28 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                     phase = QS_MINIMISE;
                 }
                 else if ((k & 1) && !qa.kicks) { // elastic: eventDrivenStep(eps, false)
-                    double m = __builtin_inf();
-                    int outside = 0;
+                    // (Smooth has no maxUniformDisplacement: the host refuses
+                    // elastic events for it, detail.h:414-422)
+                    if constexpr (POT != POT_SMOOTH) {
+                        double m = __builtin_inf();
+                        int outside = 0;
 #pragma unroll
-                    for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt) {
-                            const double x = max_uniform_particle<POT>(S, u[i], yl[i], yr[i], qa.direction, outside);
-                            m = x < m ? x : m;
+                        for (int i = 0; i < PPT; ++i) {
+                            if (i < cnt) {
+                                const double x =
+                                    max_uniform_particle<POT>(S, u[i], yl[i], yr[i], qa.direction, outside);
+                                m = x < m ? x : m;
+                            }
                         }
-                    }
-                    long long out;
-                    qs_min(outside, m, out, m);
-                    if (out) {
-                        m = 0.0;
-                    }
-                    if (!(m < 0.5 * qa.eps)) {
-                        du = qa.direction > 0 ? m - 0.5 * qa.eps : 0.5 * qa.eps - m;
-                        move = true;
+                        long long out;
+                        qs_min(outside, m, out, m);
+                        if (out) {
+                            m = 0.0;
+                        }
+                        if (!(m < 0.5 * qa.eps)) {
+                            du = qa.direction > 0 ? m - 0.5 * qa.eps : 0.5 * qa.eps - m;
+                            move = true;
+                        }
                     }
                     phase = QS_RECORD;
                 }

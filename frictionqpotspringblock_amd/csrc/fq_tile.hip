@@ -46,6 +46,39 @@ hipError_t launch_tile_decide(const SysView* views, int nsys, const int32_t* til
     return hipGetLastError();
 }
 
+bool tile_block_supported(int pot, int inter)
+{
+    (void)pot;
+    return inter == INT_NONE || inter == INT_LAPLACE1D || inter == INT_QUARTIC1D || inter == INT_QUARTICGRADIENT1D;
+}
+
+bool launch_tile_block(int pot, int inter, const SysView* views, const TileDesc* tiles, int ntiles, double* part,
+                       int32_t* tfail, const RunArgs& ra, int K, hipStream_t stream)
+{
+#define FQ_BLOCK(P, I)                                                                                         \
+    if (pot == P && inter == I) {                                                                              \
+        hipLaunchKernelGGL((k_tile_block<P, I>), dim3((unsigned)ntiles), dim3(TB_THREADS), 0, stream, views,  \
+                           tiles, part, tfail, ra, K);                                                         \
+        return true;                                                                                           \
+    }
+#define FQ_BLOCK_CHAINS(P) FQ_BLOCK(P, INT_NONE) FQ_BLOCK(P, INT_LAPLACE1D) FQ_BLOCK(P, INT_QUARTIC1D)         \
+    FQ_BLOCK(P, INT_QUARTICGRADIENT1D)
+    FQ_BLOCK_CHAINS(POT_CUSPY)
+    FQ_BLOCK_CHAINS(POT_SEMISMOOTH)
+    FQ_BLOCK_CHAINS(POT_SMOOTH)
+#undef FQ_BLOCK_CHAINS
+#undef FQ_BLOCK
+    return false;
+}
+
+hipError_t launch_tile_block_decide(const SysView* views, int nsys, const int32_t* tile_first, const double* part,
+                                    const int32_t* tfail, const RunArgs& ra, int K, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_tile_block_decide, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, stream, views, nsys,
+                       tile_first, part, tfail, ra, K);
+    return hipGetLastError();
+}
+
 bool launch_tile_finish(int pot, int inter, const SysView* views, const TileDesc* tiles, int ntiles,
                         hipStream_t stream)
 {

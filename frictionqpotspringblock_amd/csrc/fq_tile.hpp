@@ -270,6 +270,370 @@ __global__ void __launch_bounds__(64) k_tile_decide(const SysView* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Blocks of K steps per launch (chains). Each tile computes its particles
+// [p0, p1) together with a halo of K particles on each side, all in
+// registers, for up to K steps with one barrier per step (neighbour slips
+// through LDS, as in k_resident); the halo is recomputed from its state with
+// the same expressions as its owner, so the tile's particles stay exact for K
+// steps (errors from the halo's open ends move inwards one particle per step).
+// The tile records, per step, the residual partial sums of its own particles
+// and the first step s at which one of them needed a refill or became NaN,
+// as (s << 2) | TILE_FLAG_*; the halo's failures are recorded by their
+// owners. k_tile_block_decide then commits the K steps, or, if the StopList
+// stops earlier or a failure occurred, asks the next block to redo the first
+// c steps from the unchanged state buffer (deterministic, hence identical)
+// and commits those: c = s - 1 before a refill (then status REFILL, and the
+// host refills as after k_tile_step), c = s through a NaN step (then status
+// NAN, as k_tile_decide does).
+// ---------------------------------------------------------------------------
+
+constexpr int TB_THREADS = 512;
+constexpr int TB_PPT = 5;                 // particles per thread of the extended segment
+constexpr int TB_KMAX = (TB_THREADS * TB_PPT - TB_TILE_PARTICLES) / 2; // 256
+constexpr int32_t TB_NO_FAIL = 0x7fffffff;
+
+// steps the block of system S takes (the redo count, or min(K, target - steps))
+__device__ __forceinline__ int tb_steps(const RunState* rs, int K)
+{
+    const int32_t redo = rs->tb_redo;
+    if (redo > 0) {
+        return redo;
+    }
+    const int64_t room = rs->target - rs->steps;
+    return (int)(room < K ? (room > 0 ? room : 0) : K);
+}
+
+template <int POT, int INT>
+__global__ void __launch_bounds__(TB_THREADS) k_tile_block(const SysView* __restrict__ views,
+                                                           const TileDesc* __restrict__ tiles,
+                                                           double* __restrict__ part, int32_t* __restrict__ tfail,
+                                                           RunArgs ra, int K)
+{
+    constexpr int PPT = TB_PPT;
+    __shared__ double s_first[2][TB_THREADS];
+    __shared__ double s_last[2][TB_THREADS];
+    __shared__ double s_red[2][TB_THREADS / 64][2];
+    __shared__ int s_fail;
+    const TileDesc td = tiles[blockIdx.x];
+    const SysView& S = views[td.sys];
+    const RunState* const rs = S.rs;
+    const int t = threadIdx.x, T = blockDim.x, lane = t & 63, wave = t >> 6, nw = T >> 6;
+    const int32_t status = uniform_i(rs->status);
+    if (status != ST_RUNNING && status != ST_BUDGET) {
+        return;
+    }
+    const int Ke = uniform_i(tb_steps(rs, K));
+    const int cb = uniform_i(rs->cur), nb = cb ^ 1;
+    const int64_t N = S.N, p0 = td.p0, n = td.p1 - td.p0;
+    const int64_t next = n + 2 * (int64_t)K; // particles of the extended segment
+    const Landscape& L = S.ls;
+    const int64_t nch = L.nchunk;
+    const int32_t jmin = (int32_t)L.buffer;
+    const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
+    const double* const Uc = cb ? S.u_b : S.u;
+    const double* const Vc = cb ? S.v_b : S.v;
+    const double* const Ac = cb ? S.a_b : S.a;
+    double* const Un = nb ? S.u_b : S.u;
+    double* const Vn = nb ? S.v_b : S.v;
+    double* const An = nb ? S.a_b : S.a;
+    const double dt = S.dt;
+    const double c2 = 0.5 * dt * dt;
+    const double hdt = 0.5 * dt;
+    const double meta = -S.eta;
+    const double inv_m = S.inv_m;
+    const double kf = S.k_frame;
+    const double dv = ra.v_frame * dt;
+    const bool mini = ra.mode == MODE_MINIMISE;
+
+    // extended index i = t * PPT + k, global particle (p0 - K + i) mod N
+    const int64_t base = (int64_t)t * PPT;
+    const int cnt = (int)(next - base >= PPT ? PPT : (next - base > 0 ? next - base : 0));
+    const int Tlast = (int)((next - 1) / PPT); // thread holding the last particle
+    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT];
+    int32_t jl[PPT];
+    auto gidx = [&](int k) -> int64_t { // global index of local particle k (periodic)
+        int64_t q = (p0 - K + base + k) % N;
+        return q < 0 ? q + N : q;
+    };
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        u[k] = v[k] = a[k] = yl[k] = yr[k] = 0.0;
+        jl[k] = 0;
+        if (k < cnt) {
+            const int64_t q = gidx(k);
+            u[k] = Uc[q];
+            v[k] = Vc[q];
+            a[k] = Ac[q];
+            // the well from jloc and the window (another tile may be writing
+            // the cached yield positions of its own particles meanwhile; jloc
+            // is one word, either value is a well of the window, and the
+            // first walk finds the particle's well from either)
+            jl[k] = L.jloc[q];
+            yl[k] = L.y[q * nch + jl[k]];
+            yr[k] = L.y[q * nch + jl[k] + 1];
+        }
+    }
+    if (t == 0) {
+        s_fail = TB_NO_FAIL;
+    }
+    double uf = rs->u_frame;
+    __syncthreads();
+
+    int s;
+    for (s = 1; s <= Ke; ++s) {
+        const int b = s & 1;
+        double* const x = u; // position update in place: x = u + dt v + dt^2/2 a
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            x[k] = u[k] + dt * v[k] + c2 * a[k];
+        }
+        if (cnt > 0) {
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (k == 0) {
+                    s_first[b][t] = x[0];
+                }
+                if (k == cnt - 1) {
+                    s_last[b][t] = x[k];
+                }
+            }
+        }
+        __syncthreads();
+        if (mini && s > 1 && t == 0) { // residual sums of step s - 1
+            double a0 = 0.0, a1 = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                a0 += s_red[b ^ 1][w][0];
+                a1 += s_red[b ^ 1][w][1];
+            }
+            part[((int64_t)blockIdx.x * K + (s - 2)) * 2] = a0;
+            part[((int64_t)blockIdx.x * K + (s - 2)) * 2 + 1] = a1;
+        }
+        if ((uniform_i(s_fail) >> 2) < s) {
+            break; // a particle of the tile failed at an earlier step
+        }
+        if (ra.flow) {
+            uf = uf + dv;
+        }
+        double xlast = x[0]; // (selected, not indexed: no scratch)
+#pragma unroll
+        for (int k = 1; k < PPT; ++k) {
+            xlast = k == cnt - 1 ? x[k] : xlast;
+        }
+        // open ends of the extended segment: any finite value
+        const double left = (t == 0) ? x[0] : s_last[b][t - 1];
+        const double right = (t == Tlast) ? xlast : s_first[b][t + 1 < T ? t + 1 : t];
+        double sf = 0.0, sff = 0.0;
+        int fail = 0;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            if (k < cnt) {
+                const int64_t i = base + k;
+                const bool exact = i >= s - 1 && i < next - s + 1; // the state of step s - 1 is exact
+                const bool own = i >= K && i < K + n;
+                const double xk = x[k];
+                if (own && __builtin_isnan(xk)) {
+                    fail |= TILE_FLAG_NAN;
+                }
+                if (exact && !(yl[k] <= xk && xk < yr[k]) && __builtin_isfinite(xk)) {
+                    const double* row = L.y + gidx(k) * nch;
+                    int32_t j = jl[k];
+                    double lo = yl[k], hi = yr[k];
+                    while (xk >= hi && j < jmax) {
+                        ++j;
+                        lo = hi;
+                        hi = row[j + 1];
+                    }
+                    while (xk < lo && j > jmin) {
+                        --j;
+                        hi = lo;
+                        lo = row[j];
+                    }
+                    if (lo <= xk && xk < hi) {
+                        jl[k] = j;
+                        yl[k] = lo;
+                        yr[k] = hi;
+                    }
+                    else if (own) {
+                        fail |= TILE_FLAG_REFILL;
+                    }
+                }
+                double fi;
+                if constexpr (INT == INT_NONE) {
+                    fi = 0.0;
+                }
+                else {
+                    double um = 0.0, up = 0.0;
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        if (q == k) {
+                            um = (q == 0) ? left : x[q > 0 ? q - 1 : 0];
+                            up = (q == cnt - 1) ? right : x[q + 1 < PPT ? q + 1 : q];
+                        }
+                    }
+                    fi = chain_force<INT>(S, um, xk, up);
+                }
+                const double fp = potential_force<POT>(S, xk, yl[k], yr[k]);
+                const double ff = kf * (uf - xk);
+                const double sum3 = ff + fp + fi;
+                const double vn = v[k], an = a[k];
+                double vv = vn + dt * an;
+                double f = sum3 + meta * vv;
+                double aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                v[k] = vv;
+                a[k] = aa;
+                if (own) {
+                    sf += f * f;
+                    sff += ff * ff;
+                }
+            }
+        }
+        if (fail) { // a refill before a NaN of the same step
+            atomicMin(&s_fail, (s << 2) | ((fail & TILE_FLAG_REFILL) ? TILE_FLAG_REFILL : TILE_FLAG_NAN));
+        }
+        if (mini) {
+            sf = wave_sum(sf);
+            sff = wave_sum(sff);
+            if (lane == 0) {
+                s_red[b][wave][0] = sf;
+                s_red[b][wave][1] = sff;
+            }
+        }
+    }
+    const int done = s - 1;
+    __syncthreads();
+    if (t == 0) {
+        if (mini && done >= 1 && done == Ke) { // sums of the last step
+            const int lb = done & 1;
+            double a0 = 0.0, a1 = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                a0 += s_red[lb][w][0];
+                a1 += s_red[lb][w][1];
+            }
+            part[((int64_t)blockIdx.x * K + (done - 1)) * 2] = a0;
+            part[((int64_t)blockIdx.x * K + (done - 1)) * 2 + 1] = a1;
+        }
+        tfail[blockIdx.x] = s_fail;
+    }
+    // the tile's particles: state after the block into the other buffer (used
+    // only if the decision commits all Ke steps), wells in place
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int64_t i = base + k;
+        if (k < cnt && i >= K && i < K + n) {
+            const int64_t q = gidx(k);
+            Un[q] = u[k];
+            Vn[q] = v[k];
+            An[q] = a[k];
+            L.jloc[q] = jl[k];
+            S.yl_a[q] = yl[k];
+            S.yr_a[q] = yr[k];
+        }
+    }
+}
+
+// Commit (or ask for a redo of) the block of every system; one thread per
+// system. The StopList runs over the block's steps in order.
+__global__ void __launch_bounds__(64) k_tile_block_decide(const SysView* __restrict__ views, int nsys,
+                                                           const int32_t* __restrict__ tile_first,
+                                                           const double* __restrict__ part,
+                                                           const int32_t* __restrict__ tfail, RunArgs ra, int K)
+{
+    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sy >= nsys) {
+        return;
+    }
+    const SysView& S = views[sy];
+    RunState* const rs = S.rs;
+    if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
+        return;
+    }
+    const int Ke = tb_steps(rs, K);
+    int32_t code = TB_NO_FAIL; // (first failing step << 2) | TILE_FLAG_* over the tiles
+    for (int k = tile_first[sy]; k < tile_first[sy + 1]; ++k) {
+        code = tfail[k] < code ? tfail[k] : code;
+    }
+    const int fs = code >> 2, ftype = code & 3;
+    // steps that may be committed: those before a refill, through a NaN
+    const int valid = ftype == TILE_FLAG_NAN ? (fs < Ke ? fs : Ke) : (fs - 1 < Ke ? fs - 1 : Ke);
+    const int checked = ftype == TILE_FLAG_NAN ? valid - 1 : valid; // StopList steps (not the NaN step)
+    int c = valid;
+    bool conv = false;
+    int32_t below = rs->below, below2 = rs->below2, dec = rs->dec;
+    double prev = rs->prev;
+    if (ra.mode == MODE_MINIMISE) {
+        const int64_t nt = ra.niter_tol;
+        for (int st = 0; st < checked; ++st) {
+            double sf = 0.0, sff = 0.0;
+            for (int k = tile_first[sy]; k < tile_first[sy + 1]; ++k) {
+                sf += part[((int64_t)k * K + st) * 2];
+                sff += part[((int64_t)k * K + st) * 2 + 1];
+            }
+            const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
+            const double r = (rff != 0.0) ? rf / rff : rf;
+            below = !(r >= ra.tol) ? below + 1 : 0;
+            below2 = !(r >= ra.tol2) ? below2 + 1 : 0;
+            dec = !(r >= prev) ? dec + 1 : 0;
+            prev = r;
+            if (nt <= 0 || (dec >= nt - 1 && below >= nt) || below2 >= nt) {
+                c = st + 1;
+                conv = true;
+                break;
+            }
+        }
+    }
+    // the failure to report after the committed steps (none if the StopList
+    // stopped first)
+    const int32_t pending = (!conv && fs <= Ke) ? ftype : 0;
+    if (c < Ke) {
+        if (c > 0) { // the next block redoes the first c steps
+            rs->tb_redo = c;
+            rs->tb_fail = pending;
+            return;
+        }
+        // nothing to commit (a refill at the first step): report it now
+        rs->tb_redo = 0;
+        rs->tb_fail = 0;
+        rs->status = pending == TILE_FLAG_NAN ? ST_NAN : ST_REFILL;
+        return;
+    }
+    // commit the Ke steps
+    if (Ke > 0) {
+        rs->cur ^= 1;
+    }
+    rs->steps += Ke;
+    rs->inc += Ke;
+    if (ra.flow) {
+        double uf = rs->u_frame;
+        for (int st = 0; st < Ke; ++st) {
+            uf = uf + ra.v_frame * S.dt;
+        }
+        rs->u_frame = uf;
+    }
+    rs->below = below;
+    rs->below2 = below2;
+    rs->dec = dec;
+    rs->prev = prev;
+    rs->tb_redo = 0;
+    const int32_t report = rs->tb_fail ? rs->tb_fail : pending;
+    rs->tb_fail = 0;
+    if (conv) {
+        rs->status = ST_CONVERGED;
+    }
+    else if (report == TILE_FLAG_NAN) {
+        rs->status = ST_NAN;
+    }
+    else if (report == TILE_FLAG_REFILL) {
+        rs->status = ST_REFILL;
+    }
+}
+
 // The final state into the primary arrays and the forces of it (quenched if
 // the StopList converged).
 template <int POT, int INT>

@@ -25,17 +25,21 @@ def fq():
     return fq
 
 
-@pytest.fixture(params=["resident", "tiled", "streaming"])
+@pytest.fixture(params=["resident", "tiled", "tiled_step", "streaming"])
 def path(request, monkeypatch):
     """Run a test through the resident kernels, again with every system on
-    the large-system path (FQSB_FORCE_STREAMING=1: the tiled per-step kernels
-    for timeSteps / flowSteps / minimise, the one-workgroup streaming kernel
-    otherwise), and again with the tiles switched off (FQSB_NO_TILES=1: the
-    streaming kernel for everything)."""
+    the large-system path (FQSB_FORCE_STREAMING=1: the tiled kernels for
+    timeSteps / flowSteps / minimise -- blocks of 32 steps per launch for
+    chains --, the one-workgroup streaming kernel otherwise), with the tiled
+    path one step per launch (FQSB_TILE_K=0), and with the tiles switched off
+    (FQSB_NO_TILES=1: the streaming kernel for everything)."""
     monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
     monkeypatch.delenv("FQSB_NO_TILES", raising=False)
-    if request.param in ("tiled", "streaming"):
+    monkeypatch.delenv("FQSB_TILE_K", raising=False)
+    if request.param in ("tiled", "tiled_step", "streaming"):
         monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
+    if request.param == "tiled_step":
+        monkeypatch.setenv("FQSB_TILE_K", "0")
     if request.param == "streaming":
         monkeypatch.setenv("FQSB_NO_TILES", "1")
     return request.param
@@ -418,6 +422,34 @@ def test_large_systems(fq, oracle, sub, cls, n, extra, minimise):
         assert a.minimise(time_activity=True) == b.minimise(time_activity=True) == 0
         assert a.quasistaticActivityFirst == b.quasistaticActivityFirst
         assert_same_state(a, b, "kick + minimise")
+
+
+@pytest.mark.parametrize("k", ["1", "7", "256"])
+def test_tile_blocks(fq, oracle, monkeypatch, k):
+    """Blocks of K steps per launch on a large chain (halo recomputation,
+    redo after the StopList or a refill inside a block, the commit of the
+    prefix before a refill) against the oracle: flowSteps with crossings and
+    refills (60-well windows), minimise, kick + minimise."""
+    monkeypatch.setenv("FQSB_TILE_K", k)
+    par = _base(20000, k_interactions=1.0, nchunk=60)
+    a = fq.Line1d.System_Cuspy_Laplace(**par)
+    b = oracle.Line1d.System_Cuspy_Laplace(**par)
+    v0 = np.random.default_rng(5).normal(0.0, 10.0, a.shape)
+    for s in (a, b):
+        s.v = v0
+        s.flowSteps(300, 0.2)
+    assert_same_state(a, b, "flowSteps")
+    assert np.array_equal(a.chunk.data, b.chunk.data)
+    assert a.minimise() == b.minimise() == 0
+    assert_same_state(a, b, "minimise")
+    for s in (a, b):
+        s.eventDrivenStep(1e-3, False)
+        s.eventDrivenStep(1e-3, True)
+    assert a.minimise() == b.minimise() == 0
+    assert_same_state(a, b, "kick + minimise")
+    for s in (a, b):
+        s.timeSteps(45)
+    assert_same_state(a, b, "timeSteps")
 
 
 def test_tiled_graphs_match_eager(fq, monkeypatch):
