@@ -1638,7 +1638,7 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds)
             g->tile_first = dalloc<int32_t>((size_t)R + 1);
             g->h2d(g->tile_first, first.data(), (size_t)R + 1);
             g->tile_part = dalloc<double>(2 * tl.size());
-            g->tile_k = 32;
+            g->tile_k = 128;
             if (const char* e = std::getenv("FQSB_TILE_K")) {
                 g->tile_k = std::max(0, std::min(std::atoi(e), 256));
             }

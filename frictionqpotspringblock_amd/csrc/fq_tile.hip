@@ -74,8 +74,8 @@ bool launch_tile_block(int pot, int inter, const SysView* views, const TileDesc*
 hipError_t launch_tile_block_decide(const SysView* views, int nsys, const int32_t* tile_first, const double* part,
                                     const int32_t* tfail, const RunArgs& ra, int K, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_tile_block_decide, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, stream, views, nsys,
-                       tile_first, part, tfail, ra, K);
+    hipLaunchKernelGGL(k_tile_block_decide, dim3((unsigned)nsys), dim3(64), 0, stream, views, tile_first, part, tfail,
+                       ra, K);
     return hipGetLastError();
 }
 

@@ -538,45 +538,61 @@ __global__ void __launch_bounds__(TB_THREADS) k_tile_block(const SysView* __rest
     }
 }
 
-// Commit (or ask for a redo of) the block of every system; one thread per
-// system. The StopList runs over the block's steps in order.
-__global__ void __launch_bounds__(64) k_tile_block_decide(const SysView* __restrict__ views, int nsys,
+// Commit (or ask for a redo of) the block of every system: one wavefront per
+// system; lane l sums the tiles' partial norms of steps l, l + 64, ... (over
+// the tiles in order), then lane 0 runs the StopList over the steps in order.
+__global__ void __launch_bounds__(64) k_tile_block_decide(const SysView* __restrict__ views,
                                                            const int32_t* __restrict__ tile_first,
                                                            const double* __restrict__ part,
                                                            const int32_t* __restrict__ tfail, RunArgs ra, int K)
 {
-    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sy >= nsys) {
-        return;
-    }
+    __shared__ double s_r[TB_KMAX];
+    const int sy = blockIdx.x, lane = threadIdx.x;
     const SysView& S = views[sy];
     RunState* const rs = S.rs;
-    if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
+    const int32_t status = uniform_i(rs->status);
+    if (status != ST_RUNNING && status != ST_BUDGET) {
         return;
     }
-    const int Ke = tb_steps(rs, K);
+    const int Ke = uniform_i(tb_steps(rs, K));
+    const int k0 = tile_first[sy], k1 = tile_first[sy + 1];
     int32_t code = TB_NO_FAIL; // (first failing step << 2) | TILE_FLAG_* over the tiles
-    for (int k = tile_first[sy]; k < tile_first[sy + 1]; ++k) {
+    for (int k = k0 + lane; k < k1; k += 64) {
         code = tfail[k] < code ? tfail[k] : code;
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        const int32_t o = __shfl_xor(code, off, 64);
+        code = o < code ? o : code;
+    }
+    code = uniform_i(code);
     const int fs = code >> 2, ftype = code & 3;
     // steps that may be committed: those before a refill, through a NaN
     const int valid = ftype == TILE_FLAG_NAN ? (fs < Ke ? fs : Ke) : (fs - 1 < Ke ? fs - 1 : Ke);
     const int checked = ftype == TILE_FLAG_NAN ? valid - 1 : valid; // StopList steps (not the NaN step)
-    int c = valid;
-    bool conv = false;
-    int32_t below = rs->below, below2 = rs->below2, dec = rs->dec;
-    double prev = rs->prev;
-    if (ra.mode == MODE_MINIMISE) {
-        const int64_t nt = ra.niter_tol;
-        for (int st = 0; st < checked; ++st) {
+    const bool mini = ra.mode == MODE_MINIMISE;
+    if (mini) {
+        for (int st = lane; st < checked; st += 64) {
             double sf = 0.0, sff = 0.0;
-            for (int k = tile_first[sy]; k < tile_first[sy + 1]; ++k) {
+            for (int k = k0; k < k1; ++k) {
                 sf += part[((int64_t)k * K + st) * 2];
                 sff += part[((int64_t)k * K + st) * 2 + 1];
             }
             const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
-            const double r = (rff != 0.0) ? rf / rff : rf;
+            s_r[st] = (rff != 0.0) ? rf / rff : rf;
+        }
+    }
+    __syncthreads();
+    if (lane != 0) {
+        return;
+    }
+    int c = valid;
+    bool conv = false;
+    int32_t below = rs->below, below2 = rs->below2, dec = rs->dec;
+    double prev = rs->prev;
+    if (mini) {
+        const int64_t nt = ra.niter_tol;
+        for (int st = 0; st < checked; ++st) {
+            const double r = s_r[st];
             below = !(r >= ra.tol) ? below + 1 : 0;
             below2 = !(r >= ra.tol2) ? below2 + 1 : 0;
             dec = !(r >= prev) ? dec + 1 : 0;
