@@ -610,6 +610,34 @@ def run_quasistatic(args):
     }
 
 
+def rg_tile_roofline(replicas=8, n=65536, steps=2048):
+    """The ragged workload's dominant kernel on its own: the blocked tiled
+    path (k_tile_block, K steps per launch, with its decision kernel) on
+    Cuspy-Laplace chains of the workload's largest size, flowSteps at the
+    workload's parameters, timed with HIP events on the batch's stream."""
+    from frictionqpotspringblock_amd.batch import Batch
+
+    b = Batch(replicas, seed_stride=n, potential="Cuspy", interactions="Laplace1d",
+              **dict(PARAMS, k_frame=1.0 / n, shape=[n]))
+    b.flowSteps(256, V_FRAME)
+    ms = b.flowSteps(steps, V_FRAME, timed=True)
+    ps = replicas * n * steps
+    achieved = FLOP_PER_PARTICLE_STEP * ps / (ms * 1e-3) / 1e12
+    return {
+        "kernel": f"k_tile_block<Cuspy, Laplace1d> + k_tile_block_decide ({replicas} chains x {n}, "
+        "blocks of 128 steps over tiles of 2048 particles)",
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": 78.6,
+        "unit": "TFLOP/s",
+        "frac": achieved / 78.6,
+        "traffic": None,
+        "flop_per_particle_step": FLOP_PER_PARTICLE_STEP,
+        "particle_steps_per_s": ps / (ms * 1e-3),
+        "ms_per_block": ms / (steps / 128),
+    }
+
+
 def run_ragged(args):
     import torch
 
@@ -650,6 +678,7 @@ def run_ragged(args):
         work += float(np.sum(nsizes * rgn.last_steps))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    roof = rg_tile_roofline()
     cpu = None if args.no_cpu else rg_cpu_baseline()
     return {
         "metric": METRIC,
@@ -675,7 +704,7 @@ def run_ragged(args):
             "inertial_seconds": t_inertial,
             "particle_steps": work,
         },
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
 
