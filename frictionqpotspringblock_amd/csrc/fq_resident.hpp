@@ -251,6 +251,8 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
     const int Tact = (int)((N + PPT - 1) / PPT);
     const Landscape& L = S.ls;
     const bool track = ra.mode != MODE_STEPS;
+    // per-step counts of changed wells (events, activity, truncation)
+    const bool counts = ra.mode == MODE_UNTIL_EVENT || ra.mode == MODE_TRUNCATE || ra.time_activity;
 
     // hot scalars
     double* const Y = L.y;
@@ -658,8 +660,12 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                     for (int w = 0; w < nw; ++w) {
                         sf += s_red_d[pb][w][0];
                         sff += s_red_d[pb][w][1];
-                        nchg += s_red_i[pb][w][0];
-                        sabs += s_red_i[pb][w][1];
+                    }
+                    if (counts) {
+                        for (int w = 0; w < nw; ++w) {
+                            nchg += s_red_i[pb][w][0];
+                            sabs += s_red_i[pb][w][1];
+                        }
                     }
                     sf = uniform_d(sf);
                     sff = uniform_d(sff);
@@ -816,6 +822,8 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                         if (track) {
                             sf += f * f;
                             sff += ff * ff;
+                        }
+                        if (counts) {
                             nchg += dl[i] != 0;
                             sabs += dl[i] < 0 ? -dl[i] : dl[i];
                         }
@@ -824,10 +832,14 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 if (track) {
                     sf = wave_sum(sf);
                     sff = wave_sum(sff);
-                    long long n1 = wave_sum_i(nchg), n2 = wave_sum_i(sabs);
                     if (lane == 0) {
                         s_red_d[b][wave][0] = sf;
                         s_red_d[b][wave][1] = sff;
+                    }
+                }
+                if (counts) {
+                    const long long n1 = wave_sum_i(nchg), n2 = wave_sum_i(sabs);
+                    if (lane == 0) {
                         s_red_i[b][wave][0] = n1;
                         s_red_i[b][wave][1] = n2;
                     }
