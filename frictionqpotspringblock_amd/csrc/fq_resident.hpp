@@ -176,41 +176,38 @@ FQ_HD size_t resident_lds_bytes(int64_t N, int threads)
     }
 }
 
-// Landscape refill inside a resident launch: every particle p = base ..
-// base + cnt - 1 of every lane is aligned to the slip of the pending step,
-// u + dt v + dt^2/2 a, or to u itself (at_u, a uniform displacement of the
-// quasistatic driver), from the state in memory, exactly as k_refill does.
-// The particles that leave their well are taken one at a time by the whole
-// wavefront (chunk_align_wave: the new yield positions are drawn in
-// parallel). Called with the state written back, so that no registers are
-// live across the call.
-__device__ __noinline__ void refill_wave(const SysView* __restrict__ views, int64_t base, int cnt, int at_u)
+// Landscape refill inside a resident launch: particle base + i of a lane
+// with bit i of `mask` set left the window's safe zone; it is aligned to the
+// slip of the pending step, u + dt v + dt^2/2 a, or to u itself (at_u, a
+// uniform displacement of the quasistatic driver), from the state in memory,
+// exactly as k_refill does. The particles are taken one at a time by the
+// whole wavefront (chunk_align_wave: the new yield positions are drawn in
+// parallel). Called with the state written back (and a barrier after it), so
+// that no registers are live across the call.
+__device__ __noinline__ void refill_wave(const SysView* __restrict__ views, int64_t base, int at_u, uint32_t mask)
 {
     const SysView& S = views[blockIdx.x];
     const Landscape& L = S.ls;
     const double dt = S.dt;
     const double c2 = 0.5 * dt * dt;
-    int cmax = cnt; // the most particles a lane of this wavefront holds
-    for (int off = 32; off > 0; off >>= 1) {
-        const int o = __shfl_xor(cmax, off, 64);
-        cmax = o > cmax ? o : cmax;
-    }
-    for (int i = 0; i < cmax; ++i) {
-        const int64_t p = base + i;
-        double x = 0.0;
-        int32_t j = 0;
-        bool need = false;
-        if (i < cnt) {
-            x = at_u ? S.u[p] : S.u[p] + dt * S.v[p] + c2 * S.a[p];
-            j = L.jloc[p];
-            const double* row = L.y + p * L.nchunk;
-            need = !(row[j] <= x && x < row[j + 1]);
-        }
-        uint64_t pending = __ballot(need);
-        while (pending) {
-            const int src = __builtin_ctzll(pending);
-            pending &= pending - 1;
-            chunk_align_wave(L, __shfl(p, src, 64), __shfl(x, src, 64), __shfl(j, src, 64));
+    // another lane's particle, written back by that lane: read through volatile
+    // (vector) loads
+    const volatile double* const vu = S.u;
+    const volatile double* const vv = S.v;
+    const volatile double* const va = S.a;
+    const volatile int32_t* const vj = L.jloc;
+    uint64_t lanes = __ballot(mask != 0);
+    while (lanes) {
+        const int src = __builtin_ctzll(lanes);
+        lanes &= lanes - 1;
+        uint32_t m = (uint32_t)__shfl((int)mask, src, 64);
+        const int64_t b0 = __shfl(base, src, 64);
+        while (m) {
+            const int i = __builtin_ctz(m);
+            m &= m - 1;
+            const int64_t p = b0 + i;
+            const double x = at_u ? vu[p] : vu[p] + dt * vv[p] + c2 * va[p];
+            chunk_align_wave(L, p, x, vj[p]);
         }
     }
 }
@@ -293,6 +290,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         ev_done0 = rs->steps;
     }
     int64_t launch_done = 0; // steps of the earlier entries of this launch
+    uint32_t rmask = 0;      // this thread's particles that need a refill (bit i: particle base + i)
     int32_t status;
     int it;
 
@@ -470,7 +468,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         // its window; returns true if some particle left the window (the state is
         // then left for k_refill to align at u)
         auto qs_advance = [&](double du) __attribute__((always_inline)) -> bool {
-            long long need = 0;
+            uint32_t need = 0;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 if (i < cnt) {
@@ -497,15 +495,16 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                             yr[i] = hi;
                         }
                         else {
-                            need = 1;
+                            need |= 1u << i;
                         }
                     }
                 }
             }
             uf = uf + du * (kf + S.mu) / kf;
+            rmask = need;
             long long any;
             double unused;
-            qs_sum(need, 0.0, any, unused);
+            qs_sum(need != 0, 0.0, any, unused);
             return any != 0;
         };
 
@@ -632,16 +631,19 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 // its window; otherwise step `it` is left to a refill and a relaunch
                 publish(b, true);
                 {
-                    int refill = 0;
+                    uint32_t refill = 0; // particles whose next well is outside the safe zone
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
                         if (i < cnt) {
                             const double x = u[i] + dt * v[i] + c2 * a[i];
                             if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
-                                refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax);
+                                if (!reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax)) {
+                                    refill |= 1u << i;
+                                }
                             }
                         }
                     }
+                    rmask = refill;
                     if (refill) {
                         s_refill[b] = it;
                     }
@@ -931,7 +933,8 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         }
         // refill from the written-back state, then reload it: the LDS of this
         // entry must no longer be read when the next one publishes
-        refill_wave(views, base, cnt, refill_u);
+        __syncthreads(); // the state of every lane's particles is in memory
+        refill_wave(views, base, refill_u, rmask);
         refill_u = 0;
         __syncthreads();
     }
