@@ -19,14 +19,14 @@ int adj_variants(int* ppt, int* maxt, int cap)
     return n;
 }
 
-bool launch_adj_record(int ppt, int threads, const SysView* views, const AdjBuffers& ab, int steps, int flow,
+bool launch_adj_record(int ppt, int threads, const SysView* views, const AdjBuffers& ab, int windows, int flow,
                        double v_frame, hipStream_t stream)
 {
     switch (ppt) {
 #define FQ_REC(P, M)                                                                                           \
     case P:                                                                                                    \
-        hipLaunchKernelGGL((k_adj_record<P, M>), dim3((unsigned)ab.R), dim3(threads), 0, stream, views, ab,     \
-                           steps, flow, v_frame);                                                              \
+        hipLaunchKernelGGL((k_adj_record<P, M>), dim3((unsigned)ab.R, (unsigned)windows), dim3(threads), 0,     \
+                           stream, views, ab, flow, v_frame);                                                  \
         return true;
         FQ_ADJ_CASES(FQ_REC)
 #undef FQ_REC

@@ -26,14 +26,29 @@
 namespace fq {
 
 
-// Record one window of `steps` steps from the work state.
+// Record windows w0 + blockIdx.y (each of min(K, n - w K) steps) from their
+// start states, one workgroup per replica and window: every window is
+// independent given its checkpoint, so all of a group run at once.
 template <int PPT, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__ views, AdjBuffers ab, int steps,
-                                                     int flow, double v_frame)
+__global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__ views, AdjBuffers ab, int flow,
+                                                     double v_frame)
 {
     __shared__ double h_first[2][MAXT];
     __shared__ double h_last[2][MAXT];
     const int r = blockIdx.x;
+    const int64_t w = ab.w0 + blockIdx.y;
+    const int steps = (int)(ab.n - w * ab.K < ab.K ? ab.n - w * ab.K : ab.K);
+    // this window's start state and tape
+    ab.wu += blockIdx.y * ab.ws;
+    ab.wv += blockIdx.y * ab.ws;
+    ab.wa += blockIdx.y * ab.ws;
+    ab.wj += blockIdx.y * ab.ws;
+    ab.wuf += blockIdx.y * ab.R;
+    ab.tu += blockIdx.y * ab.ts;
+    ab.tv += blockIdx.y * ab.ts;
+    ab.ta += blockIdx.y * ab.ts;
+    ab.tj += blockIdx.y * ab.ts;
+    ab.tuf += blockIdx.y * ab.K * ab.R;
     const SysView& S = views[r];
     const int t = threadIdx.x;
     const int64_t N = ab.N;
@@ -138,16 +153,6 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                 v[i] = vv;
                 a[i] = aa;
             }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        if (i < cnt) {
-            const int64_t p = base + i;
-            ab.wu[off + p] = u[i];
-            ab.wv[off + p] = v[i];
-            ab.wa[off + p] = a[i];
-            ab.wj[off + p] = jl[i];
         }
     }
     if (bad) {

@@ -83,6 +83,8 @@ void dfree(T*& p)
 
 // steps per resident launch of a minimise-type run (bounds one kernel's duration)
 constexpr uint64_t MIN_BLOCK = 1u << 18;
+// adjoint: device memory for the tape of the windows recorded by one launch
+constexpr size_t ADJ_TAPE_BYTES = size_t(16) << 30;
 // In ragged sets, systems above this size run on the tiled path (many
 // workgroups per system): with a few hundred systems the resident path
 // leaves a large system to one workgroup, the tiled path spreads it over the
@@ -110,7 +112,7 @@ struct Group {
     int device = 0;
     int64_t refills = 0; // landscape refills so far (the adjoint needs runs without)
     // adjoint (fq_adjoint.hpp): checkpoints of the last adjoint_forward, tape
-    // of one window, work and adjoint state
+    // of a group of windows, adjoint state
     int64_t adj_n = 0, adj_K = 0, adj_nw = 0;
     int adj_flow = 0;
     double adj_vf = 0.0;
@@ -120,8 +122,7 @@ struct Group {
     std::vector<double> ck_uf; // [nw][R]
     double *tp_u = nullptr, *tp_v = nullptr, *tp_a = nullptr, *tp_uf = nullptr;
     int32_t* tp_j = nullptr;
-    double *wk_u = nullptr, *wk_v = nullptr, *wk_a = nullptr, *wk_uf = nullptr;
-    int32_t* wk_j = nullptr;
+    double* ck_uf_d = nullptr; // device copy of ck_uf
     double *ad_u = nullptr, *ad_v = nullptr, *ad_a = nullptr, *ad_par = nullptr;
     int32_t* ad_err = nullptr;
     int64_t ck_cap = 0, tp_cap = 0; // windows / steps the buffers hold
@@ -625,12 +626,12 @@ struct Group {
 
     void free_adjoint()
     {
-        for (double** q : {&ck_u, &ck_v, &ck_a, &tp_u, &tp_v, &tp_a, &tp_uf, &wk_u, &wk_v, &wk_a, &wk_uf, &ad_u,
-                           &ad_v, &ad_a, &ad_par}) {
+        for (double** q : {&ck_u, &ck_v, &ck_a, &tp_u, &tp_v, &tp_a, &tp_uf, &ck_uf_d, &ad_u, &ad_v, &ad_a,
+                           &ad_par}) {
             dfree(*q);
             *q = nullptr;
         }
-        for (int32_t** q : {&ck_j, &tp_j, &wk_j, &ad_err}) {
+        for (int32_t** q : {&ck_j, &tp_j, &ad_err}) {
             dfree(*q);
             *q = nullptr;
         }
@@ -652,11 +653,6 @@ struct Group {
         ab.ta = tp_a;
         ab.tj = tp_j;
         ab.tuf = tp_uf;
-        ab.wu = wk_u;
-        ab.wv = wk_v;
-        ab.wa = wk_a;
-        ab.wj = wk_j;
-        ab.wuf = wk_uf;
         ab.au = ad_u;
         ab.av = ad_v;
         ab.aa = ad_a;
@@ -738,31 +734,35 @@ struct Group {
         }
         adjoint_check();
         const size_t RN = (size_t)total;
-        if (adj_K > tp_cap) {
+        // the tape of a group of windows, recorded by one launch (every window
+        // starts from its checkpoint): as many windows as fit in ADJ_TAPE_BYTES
+        const size_t per_window = RN * (size_t)adj_K * (3 * sizeof(double) + sizeof(int32_t)) +
+                                  (size_t)(R * adj_K) * sizeof(double);
+        const int64_t G = std::max<int64_t>(1, std::min<int64_t>(adj_nw, (int64_t)(ADJ_TAPE_BYTES / per_window)));
+        if (G * adj_K > tp_cap) {
             dfree(tp_u);
             dfree(tp_v);
             dfree(tp_a);
             dfree(tp_j);
             dfree(tp_uf);
-            tp_u = dalloc<double>(RN * (size_t)adj_K);
-            tp_v = dalloc<double>(RN * (size_t)adj_K);
-            tp_a = dalloc<double>(RN * (size_t)adj_K);
-            tp_j = dalloc<int32_t>(RN * (size_t)adj_K);
-            tp_uf = dalloc<double>((size_t)(R * adj_K));
-            tp_cap = adj_K;
+            const size_t steps = (size_t)(G * adj_K);
+            tp_u = dalloc<double>(RN * steps);
+            tp_v = dalloc<double>(RN * steps);
+            tp_a = dalloc<double>(RN * steps);
+            tp_j = dalloc<int32_t>(RN * steps);
+            tp_uf = dalloc<double>((size_t)R * steps);
+            tp_cap = G * adj_K;
         }
-        if (!wk_u) {
-            wk_u = dalloc<double>(RN);
-            wk_v = dalloc<double>(RN);
-            wk_a = dalloc<double>(RN);
-            wk_j = dalloc<int32_t>(RN);
-            wk_uf = dalloc<double>((size_t)R);
+        if (!ad_u) {
             ad_u = dalloc<double>(RN);
             ad_v = dalloc<double>(RN);
             ad_a = dalloc<double>(RN);
             ad_par = dalloc<double>((size_t)R * 4);
             ad_err = dalloc<int32_t>(1);
         }
+        dfree(ck_uf_d);
+        ck_uf_d = dalloc<double>(ck_uf.size());
+        h2d(ck_uf_d, ck_uf.data(), ck_uf.size());
         // launch shape: fewest particles per thread that fits
         int vp[8], vm[8];
         const int nv = fq::adj_variants(vp, vm, 8);
@@ -787,36 +787,47 @@ struct Group {
         HIPC(hipMemcpyAsync(ad_u, gu, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
         HIPC(hipMemcpyAsync(ad_v, gv, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
         HIPC(hipMemcpyAsync(ad_a, ga, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
-        const fq::AdjBuffers ab = adj_buffers(gy);
         for (auto& e : eva) {
             if (!e) {
                 HIPC(hipEventCreate(&e));
             }
         }
         double rec_ms = 0.0, rev_ms = 0.0;
-        for (int64_t w = adj_nw - 1; w >= 0; --w) {
-            const size_t o = (size_t)w * RN;
-            const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
-            HIPC(hipMemcpyAsync(wk_u, ck_u + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
-            HIPC(hipMemcpyAsync(wk_v, ck_v + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
-            HIPC(hipMemcpyAsync(wk_a, ck_a + o, sizeof(double) * RN, hipMemcpyDeviceToDevice, stream));
-            HIPC(hipMemcpyAsync(wk_j, ck_j + o, sizeof(int32_t) * RN, hipMemcpyDeviceToDevice, stream));
-            HIPC(hipMemcpyAsync(wk_uf, ck_uf.data() + w * R, sizeof(double) * (size_t)R, hipMemcpyHostToDevice,
-                                stream));
+        for (int64_t w1 = adj_nw - 1; w1 >= 0; w1 -= G) {
+            const int64_t w0 = std::max<int64_t>(0, w1 - G + 1);
+            fq::AdjBuffers ab = adj_buffers(gy);
+            ab.wu = ck_u + (size_t)w0 * RN;
+            ab.wv = ck_v + (size_t)w0 * RN;
+            ab.wa = ck_a + (size_t)w0 * RN;
+            ab.wj = ck_j + (size_t)w0 * RN;
+            ab.wuf = ck_uf_d + (size_t)(w0 * R);
+            ab.ws = (int64_t)RN;
+            ab.ts = adj_K * (int64_t)RN;
+            ab.K = adj_K;
+            ab.n = adj_n;
+            ab.w0 = w0;
             HIPC(hipEventRecord(eva[0], stream));
-            if (!fq::launch_adj_record(ppt, threads, views, ab, nsteps, adj_flow, adj_vf, stream)) {
+            if (!fq::launch_adj_record(ppt, threads, views, ab, (int)(w1 - w0 + 1), adj_flow, adj_vf, stream)) {
                 throw Error("adjoint record kernel not available");
             }
             HIPC(hipGetLastError());
             HIPC(hipEventRecord(eva[1], stream));
-            if (!fq::launch_adj_reverse(ppt, threads, views, ab, nsteps, stream)) {
-                throw Error("adjoint reverse kernel not available");
+            for (int64_t w = w1; w >= w0; --w) {
+                fq::AdjBuffers abw = ab;
+                const size_t to = (size_t)(w - w0) * (size_t)adj_K;
+                abw.tu = tp_u + to * RN;
+                abw.tv = tp_v + to * RN;
+                abw.ta = tp_a + to * RN;
+                abw.tj = tp_j + to * RN;
+                abw.tuf = tp_uf + to * (size_t)R;
+                const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
+                if (!fq::launch_adj_reverse(ppt, threads, views, abw, nsteps, stream)) {
+                    throw Error("adjoint reverse kernel not available");
+                }
+                HIPC(hipGetLastError());
             }
-            HIPC(hipGetLastError());
             HIPC(hipEventRecord(eva[2], stream));
-            // the host-side frame positions of the next window's copy must stay
-            // alive until it ran: synchronise per window
-            HIPC(hipStreamSynchronize(stream));
+            HIPC(hipEventSynchronize(eva[2]));
             float e0 = 0.0f, e1 = 0.0f;
             HIPC(hipEventElapsedTime(&e0, eva[0], eva[1]));
             HIPC(hipEventElapsedTime(&e1, eva[1], eva[2]));

@@ -191,7 +191,7 @@ struct AdjBuffers {
     double* ta;     // tape: a
     int32_t* tj;    // tape: chunk-local well index used by step n
     double* tuf;    // tape [K][R]: frame position of step n
-    double* wu;     // work state [R*N] (checkpoint in, end of window out)
+    double* wu;     // state at the window start [R*N] (its checkpoint)
     double* wv;
     double* wa;
     int32_t* wj;
@@ -204,6 +204,13 @@ struct AdjBuffers {
     int32_t* err;   // set to 1 if a well walk left the landscape window
     int64_t N;
     int64_t R;
+    // k_adj_record over several windows at once (workgroup row blockIdx.y =
+    // window w0 + blockIdx.y): strides between consecutive windows of the
+    // start state (elements of wu / wj; of wuf: R) and of the tape (elements
+    // of tu / tj; of tuf: K R), the window length K and the run length n
+    int64_t ws, ts;
+    int64_t K, n;
+    int64_t w0;
 };
 
 struct RunArgs {
