@@ -389,6 +389,27 @@ def test_errors(fq):
         s.f_neighbours
 
 
+@pytest.mark.parametrize("start", ["nan", "inf_velocity"])
+def test_nan_every_path(fq, oracle, path, start):
+    """A NaN (in u from the start, or grown from an infinite velocity) stops
+    timeSteps with "NaN entries found" after the same step as the oracle, on
+    every path (for the blocked tiles: the committed prefix, then the NaN
+    step)."""
+    systems = [_make(m, "Line1d", "System_Cuspy_Laplace", 300, dict(k_interactions=1.0)) for m in (fq, oracle)]
+    for s in systems:
+        if start == "nan":
+            u = s.u.copy()
+            u[7] = np.nan
+            s.u = u
+        else:
+            v = s.v.copy()
+            v[150] = np.inf
+            s.v = v
+        with pytest.raises(RuntimeError, match="NaN entries found"):
+            s.timeSteps(200)
+    assert systems[0].inc == systems[1].inc > 0
+
+
 # systems beyond the capacity of a resident workgroup (streaming path)
 LARGE = [
     ("Line1d", "System_Cuspy_Laplace", 20000, dict(k_interactions=1.0), True),
