@@ -8,6 +8,8 @@
 //   minimise (inertial) detail.h:1754-1785              (MODE_MINIMISE)
 //   minimise_truncate   detail.h:1858-1886              (MODE_TRUNCATE)
 //   RandomNormalForcing::force detail.h:930-943         (EXT = 1)
+// and, as k_quasi, the examples' quasistatic event loop around eventDrivenStep
+// (detail.h:1933-1960) and minimise (examples/Line1d_Cuspy_Laplace.py:39-61).
 //
 // One __syncthreads per step. Step `it` is started speculatively (new u into
 // registers and the LDS exchange buffer it&1) before the stop decision for
@@ -628,7 +630,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 const int b = it & 1;
                 // A: publish the slips after the position update of step `it`, and
                 // check (read only) that every particle's new well is reachable inside
-                // its window; otherwise step `it` is left to a refill and a relaunch
+                // its window; otherwise step `it` waits for an in-launch refill
                 publish(b, true);
                 {
                     uint32_t refill = 0; // particles whose next well is outside the safe zone
@@ -972,7 +974,7 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
 }
 
 // Quasistatic driver: the events of QsArgs for every system, one workgroup
-// per system, relaunched after refills and exhausted step budgets.
+// per system, relaunched when its step budget runs out.
 template <int POT, int INT, int PPT, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT <= 256 && PPT <= 8 ? 2 : 1)))
 k_quasi(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
