@@ -200,13 +200,14 @@ def test_ragged(fq, oracle):
         ("Line1d", "System_SemiSmooth_Laplace", 700, dict(k_interactions=1.0, kappa=1.0)),
         ("Line1d", "System_Cuspy_Quartic", 1500, dict(a1=1.0, a2=5.0)),
         ("Line2d", "System_Cuspy_Laplace", (16, 16), dict(k_interactions=1.0)),
+        ("Particles", "System_SemiSmooth", 400, dict(kappa=1.0)),
     ]
     systems = []
     for i, (sub, cls, n, extra) in enumerate(specs):
         shape = list(n) if isinstance(n, tuple) else [n]
         systems.append((f"{sub}.{cls}", _base(int(np.prod(shape)), shape=shape, seed=10 + i, **extra)))
     rg = Ragged(systems)
-    active = np.array([1, 1, 0, 1], dtype=bool)
+    active = np.array([1, 1, 0, 1, 1], dtype=bool)
     q = rg.quasistatic(1e-3, 15, active=active)
     u = rg.u
     for i, (sub, cls, n, extra) in enumerate(specs):
