@@ -6,13 +6,13 @@ namespace fq {
 
 // (particles per thread, threads per workgroup) variants; the host picks the
 // first one that holds the system
-#define FQ_ADJ_CASES(X) X(1, 1024) X(2, 1024) X(4, 1024)
+#define FQ_ADJ_CASES(X) X(1, 1024) X(2, 1024) X(4, 1024) X(8, 512)
 
 int adj_variants(int* ppt, int* maxt, int cap)
 {
-    const int P[] = {1, 2, 4}, M[] = {1024, 1024, 1024};
+    const int P[] = {1, 2, 4, 8}, M[] = {1024, 1024, 1024, 512};
     int n = 0;
-    for (int i = 0; i < 3 && n < cap; ++i, ++n) {
+    for (int i = 0; i < 4 && n < cap; ++i, ++n) {
         ppt[n] = P[i];
         maxt[n] = M[i];
     }

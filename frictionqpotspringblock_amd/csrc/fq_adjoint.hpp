@@ -14,10 +14,12 @@
 // A run of n steps is split into windows of K steps. For each window, last
 // to first: k_adj_record re-runs the window from its checkpoint (the same
 // expressions in the same order as the resident kernel, so the same
-// trajectory and the same wells) and writes a tape of the state at the start
-// of every step and the well used by it; k_adj_reverse then walks the tape
-// backwards. One workgroup per replica, PPT consecutive particles per thread,
-// neighbours exchanged through LDS.
+// trajectory and the same wells) and writes a tape of u1, v and a of every
+// step and the well used by it; k_adj_reverse then walks the tape backwards,
+// loading the tape of each step while it finishes the one after it. One
+// workgroup per replica, PPT consecutive particles per thread, neighbours
+// exchanged through LDS; the tape is stored so that each of a wavefront's
+// tape loads and stores is contiguous (AdjBuffers).
 #pragma once
 
 #include "fq_physics.hpp"
@@ -53,7 +55,6 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     const int t = threadIdx.x;
     const int64_t N = ab.N;
     const int64_t off = (int64_t)r * N;
-    const int64_t RN = ab.R * N;
     const int64_t base = (int64_t)t * PPT;
     const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
     const int Tact = (int)((N + PPT - 1) / PPT);
@@ -62,6 +63,9 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     const int64_t nch = L.nchunk;
     const int32_t jmin = (int32_t)L.buffer;
     const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
+    // tape entry of register 0 of this thread in step 0 (register i: + 64 i)
+    const int64_t Rnp = ab.R * ab.np;
+    const int64_t tq = (int64_t)r * ab.np + (int64_t)(t >> 6) * 64 * PPT + (t & 63);
 
     double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT];
     int32_t jl[PPT];
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
         if (flow) {
             uf = uf + dv;
         }
-        const int64_t tn = (int64_t)n * RN + off;
+        const int64_t tn = (int64_t)n * Rnp + tq;
         if (t == 0) {
             ab.tuf[(int64_t)n * ab.R + r] = uf;
         }
@@ -97,10 +101,11 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
         for (int i = 0; i < PPT; ++i) {
             if (i < cnt) {
                 const int64_t p = base + i;
-                ab.tu[tn + p] = u[i];
-                ab.tv[tn + p] = v[i];
-                ab.ta[tn + p] = a[i];
+                const int64_t q = tn + 64 * i;
+                ab.tv[q] = v[i];
+                ab.ta[q] = a[i];
                 u[i] = u[i] + dt * v[i] + c2 * a[i];
+                ab.tu[q] = u[i];
                 const double x = u[i];
                 if (!(yl[i] <= x && x < yr[i])) {
                     const double* row = Y + p * nch;
@@ -121,12 +126,17 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                     yl[i] = lo;
                     yr[i] = hi;
                 }
-                ab.tj[tn + p] = jl[i];
+                ab.tj[q] = jl[i];
             }
         }
         if (cnt > 0) {
             h_first[b][t] = u[0];
-            h_last[b][t] = u[cnt - 1];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
+                    h_last[b][t] = u[i];
+                }
+            }
         }
         __syncthreads();
         const double left = h_last[b][t == 0 ? Tact - 1 : t - 1];
@@ -180,12 +190,13 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
     const int64_t N = ab.N;
     const int64_t off = (int64_t)r * N;
-    const int64_t RN = ab.R * N;
     const int64_t base = (int64_t)t * PPT;
     const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
     const int Tact = (int)((N + PPT - 1) / PPT);
     const double* const Y = S.ls.y;
     const int64_t nch = S.ls.nchunk;
+    const int64_t Rnp = ab.R * ab.np;
+    const int64_t tq = (int64_t)r * ab.np + (int64_t)(t >> 6) * 64 * PPT + (t & 63);
 
     const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, w = S.inv_m, kf = S.k_frame;
     const double mu = S.mu, k = S.k;
@@ -210,6 +221,49 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
     }
     double gmu = 0.0, gk = 0.0, gkf = 0.0, geta = 0.0;
 
+    // the tape of the step being reversed (u1, v, a, well, frame position),
+    // loaded while the step before it in the walk is finished: u1 as soon as
+    // it has been read (at the top of the step), the rest once the first
+    // half of the step has used it
+    double pu[PPT], pv[PPT], pa[PPT];
+    int32_t pj[PPT];
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        pu[i] = pv[i] = pa[i] = 0.0;
+        pj[i] = 0;
+    }
+    double puf = 0.0;
+    auto load_u1 = [&](int n) {
+        const int64_t tn = (int64_t)n * Rnp + tq;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                pu[i] = ab.tu[tn + 64 * i];
+            }
+        }
+    };
+    auto load_rest = [&](int n) {
+        const int64_t tn = (int64_t)n * Rnp + tq;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            if (i < cnt) {
+                const int64_t q = tn + 64 * i;
+                pv[i] = ab.tv[q];
+                pa[i] = ab.ta[q];
+                pj[i] = ab.tj[q];
+            }
+        }
+        puf = ab.tuf[(int64_t)n * ab.R + r];
+    };
+    if (steps > 0) {
+        load_u1(steps - 1);
+        load_rest(steps - 1);
+    }
+    // every load above complete: the loop's wait counts then concern only
+    // the tape loads of the step before, which stay in flight until the top
+    // of the next iteration
+    __builtin_amdgcn_s_waitcnt(0);
+
     auto flush = [&](int i) {
         if (ab.gy && jc[i] >= 0) {
             double* row = ab.gy + (off + base + i) * nch;
@@ -220,24 +274,27 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
     };
 
     for (int n = steps - 1; n >= 0; --n) {
-        const int64_t tn = (int64_t)n * RN + off;
-        const double uf = ab.tuf[(int64_t)n * ab.R + r];
+        const double uf = puf;
         double x[PPT]; // u1 of the step, then the cotangent of g
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            x[i] = 0.0;
-            if (i < cnt) {
-                const int64_t p = base + i;
-                x[i] = ab.tu[tn + p] + dt * ab.tv[tn + p] + c2 * ab.ta[tn + p];
-            }
+            x[i] = pu[i];
         }
         if (cnt > 0) {
             h_first[t] = x[0];
-            h_last[t] = x[cnt - 1];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
+                    h_last[t] = x[i];
+                }
+            }
         }
         __syncthreads();
         const double left = h_last[t == 0 ? Tact - 1 : t - 1];
         const double right = h_first[t == Tact - 1 ? 0 : t + 1];
+        if (n > 0) {
+            load_u1(n - 1);
+        }
         double xm = left; // u1 of the particle before i (its original value)
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
@@ -247,14 +304,14 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
                 const double up = i == cnt - 1 ? right : x[i + 1 < PPT ? i + 1 : i];
                 const double lu = xm - 2 * u1 + up;
                 xm = u1;
-                const int32_t j = ab.tj[tn + p];
+                const int32_t j = pj[i];
                 if (j != jc[i]) {
                     flush(i);
                     jc[i] = j;
                     ymc[i] = 0.5 * (Y[p * nch + j] + Y[p * nch + j + 1]);
                 }
                 const double ym = ymc[i];
-                const double v = ab.tv[tn + p], a = ab.ta[tn + p];
+                const double v = pv[i], a = pa[i];
                 // forward quantities of the step
                 const double g = kf * (uf - u1) + (ym - u1) * mu + lu * k;
                 const double v1 = v + dt * a;
@@ -291,9 +348,17 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
                 x[i] = G;
             }
         }
+        if (n > 0) {
+            load_rest(n - 1);
+        }
         if (cnt > 0) {
             g_first[t] = x[0];
-            g_last[t] = x[cnt - 1];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
+                    g_last[t] = x[i];
+                }
+            }
         }
         __syncthreads();
         const double gleft = g_last[t == 0 ? Tact - 1 : t - 1];

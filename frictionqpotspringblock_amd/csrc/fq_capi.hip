@@ -125,7 +125,8 @@ struct Group {
     double* ck_uf_d = nullptr; // device copy of ck_uf
     double *ad_u = nullptr, *ad_v = nullptr, *ad_a = nullptr, *ad_par = nullptr;
     int32_t* ad_err = nullptr;
-    int64_t ck_cap = 0, tp_cap = 0; // windows / steps the buffers hold
+    int64_t ck_cap = 0;  // windows the checkpoint buffers hold
+    size_t tp_cap = 0;   // entries each tape buffer holds
     hipEvent_t eva[3] = {nullptr, nullptr, nullptr}; // around record and reverse launches
     float adj_record_ms = 0.0f, adj_reverse_ms = 0.0f; // per launch, last backward
     int pot = 0, inter = 0, ext = 0, minim = 0;
@@ -635,7 +636,8 @@ struct Group {
             dfree(*q);
             *q = nullptr;
         }
-        ck_cap = tp_cap = 0;
+        ck_cap = 0;
+        tp_cap = 0;
         adj_valid = false;
         for (auto& e : eva) {
             if (e) {
@@ -734,40 +736,17 @@ struct Group {
         }
         adjoint_check();
         const size_t RN = (size_t)total;
-        // the tape of a group of windows, recorded by one launch (every window
-        // starts from its checkpoint): as many windows as fit in ADJ_TAPE_BYTES
-        const size_t per_window = RN * (size_t)adj_K * (3 * sizeof(double) + sizeof(int32_t)) +
-                                  (size_t)(R * adj_K) * sizeof(double);
-        const int64_t G = std::max<int64_t>(1, std::min<int64_t>(adj_nw, (int64_t)(ADJ_TAPE_BYTES / per_window)));
-        if (G * adj_K > tp_cap) {
-            dfree(tp_u);
-            dfree(tp_v);
-            dfree(tp_a);
-            dfree(tp_j);
-            dfree(tp_uf);
-            const size_t steps = (size_t)(G * adj_K);
-            tp_u = dalloc<double>(RN * steps);
-            tp_v = dalloc<double>(RN * steps);
-            tp_a = dalloc<double>(RN * steps);
-            tp_j = dalloc<int32_t>(RN * steps);
-            tp_uf = dalloc<double>((size_t)R * steps);
-            tp_cap = G * adj_K;
-        }
-        if (!ad_u) {
-            ad_u = dalloc<double>(RN);
-            ad_v = dalloc<double>(RN);
-            ad_a = dalloc<double>(RN);
-            ad_par = dalloc<double>((size_t)R * 4);
-            ad_err = dalloc<int32_t>(1);
-        }
-        dfree(ck_uf_d);
-        ck_uf_d = dalloc<double>(ck_uf.size());
-        h2d(ck_uf_d, ck_uf.data(), ck_uf.size());
-        // launch shape: fewest particles per thread that fits
+        // launch shape: fewest particles per thread that fits, except that
+        // chains above 3072 particles take 8 per thread on 512 threads (at
+        // N = 4096, k_adj_reverse takes 0.51 ms per 100-step window of 64
+        // replicas this way, 0.59 ms with 4 per thread on 1024 threads)
         int vp[8], vm[8];
         const int nv = fq::adj_variants(vp, vm, 8);
         int ppt = 0, threads = 0;
         for (int i = 0; i < nv; ++i) {
+            if ((vp[i] == 8) != (N > 3072)) {
+                continue;
+            }
             const int64_t need = (N + vp[i] - 1) / vp[i];
             const int thr = (int)(((need + 63) / 64) * 64);
             if (thr <= vm[i]) {
@@ -779,6 +758,39 @@ struct Group {
         if (!ppt) {
             throw Error("no adjoint kernel for this system size");
         }
+        // tape entries per step (each replica's block padded to whole
+        // wavefronts of the launch shape)
+        const int64_t np = fq::adj_tape_stride(N, ppt);
+        const size_t RNt = (size_t)(R * np);
+        // the tape of a group of windows, recorded by one launch (every window
+        // starts from its checkpoint): as many windows as fit in ADJ_TAPE_BYTES
+        const size_t per_window = RNt * (size_t)adj_K * (3 * sizeof(double) + sizeof(int32_t)) +
+                                  (size_t)(R * adj_K) * sizeof(double);
+        const int64_t G = std::max<int64_t>(1, std::min<int64_t>(adj_nw, (int64_t)(ADJ_TAPE_BYTES / per_window)));
+        if ((size_t)(G * adj_K) * RNt > tp_cap) {
+            dfree(tp_u);
+            dfree(tp_v);
+            dfree(tp_a);
+            dfree(tp_j);
+            dfree(tp_uf);
+            const size_t steps = (size_t)(G * adj_K);
+            tp_u = dalloc<double>(RNt * steps);
+            tp_v = dalloc<double>(RNt * steps);
+            tp_a = dalloc<double>(RNt * steps);
+            tp_j = dalloc<int32_t>(RNt * steps);
+            tp_uf = dalloc<double>((size_t)R * steps);
+            tp_cap = steps * RNt;
+        }
+        if (!ad_u) {
+            ad_u = dalloc<double>(RN);
+            ad_v = dalloc<double>(RN);
+            ad_a = dalloc<double>(RN);
+            ad_par = dalloc<double>((size_t)R * 4);
+            ad_err = dalloc<int32_t>(1);
+        }
+        dfree(ck_uf_d);
+        ck_uf_d = dalloc<double>(ck_uf.size());
+        h2d(ck_uf_d, ck_uf.data(), ck_uf.size());
         if (gy) {
             HIPC(hipMemsetAsync(gy, 0, sizeof(double) * RN * (size_t)nchunk, stream));
         }
@@ -802,7 +814,8 @@ struct Group {
             ab.wj = ck_j + (size_t)w0 * RN;
             ab.wuf = ck_uf_d + (size_t)(w0 * R);
             ab.ws = (int64_t)RN;
-            ab.ts = adj_K * (int64_t)RN;
+            ab.ts = adj_K * (int64_t)RNt;
+            ab.np = np;
             ab.K = adj_K;
             ab.n = adj_n;
             ab.w0 = w0;
@@ -815,10 +828,10 @@ struct Group {
             for (int64_t w = w1; w >= w0; --w) {
                 fq::AdjBuffers abw = ab;
                 const size_t to = (size_t)(w - w0) * (size_t)adj_K;
-                abw.tu = tp_u + to * RN;
-                abw.tv = tp_v + to * RN;
-                abw.ta = tp_a + to * RN;
-                abw.tj = tp_j + to * RN;
+                abw.tu = tp_u + to * RNt;
+                abw.tv = tp_v + to * RNt;
+                abw.ta = tp_a + to * RNt;
+                abw.tj = tp_j + to * RNt;
                 abw.tuf = tp_uf + to * (size_t)R;
                 const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
                 if (!fq::launch_adj_reverse(ppt, threads, views, abw, nsteps, stream)) {

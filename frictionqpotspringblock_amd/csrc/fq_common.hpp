@@ -184,11 +184,15 @@ struct LrBuffers {
 };
 
 // Buffers of the adjoint of a batch (fq_adjoint.hpp); [R*N] arrays are
-// replica after replica.
+// replica after replica. The tape holds, per step, R blocks of np entries
+// (np = N rounded up to 64 PPT): particle p = 64 PPT w + PPT l + i (wavefront
+// w, lane l, register i of the kernels' launch shape) is entry
+// 64 PPT w + 64 i + l of its replica's block, so that a wavefront's load of
+// one register is contiguous.
 struct AdjBuffers {
-    double* tu;     // tape [K][R*N]: u at the start of step n
-    double* tv;     // tape: v
-    double* ta;     // tape: a
+    double* tu;     // tape [K][R*np]: u1 = u + dt v + dt^2/2 a of step n
+    double* tv;     // tape: v at the start of step n
+    double* ta;     // tape: a at the start of step n
     int32_t* tj;    // tape: chunk-local well index used by step n
     double* tuf;    // tape [K][R]: frame position of step n
     double* wu;     // state at the window start [R*N] (its checkpoint)
@@ -211,7 +215,14 @@ struct AdjBuffers {
     int64_t ws, ts;
     int64_t K, n;
     int64_t w0;
+    int64_t np; // tape entries per replica and step
 };
+
+// tape entries per replica and step for PPT particles per thread
+FQ_HD constexpr int64_t adj_tape_stride(int64_t N, int ppt)
+{
+    return (N + 64 * ppt - 1) / (64 * ppt) * (64 * ppt);
+}
 
 struct RunArgs {
     int32_t mode;

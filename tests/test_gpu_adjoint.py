@@ -61,13 +61,16 @@ def _setup(b, rng, vscale):
 
 
 @pytest.mark.parametrize("flow", [False, True])
-def test_dot_product_with_crossings(flow):
+@pytest.mark.parametrize("N", [200, 1500, 2100, 3500])
+def test_dot_product_with_crossings(flow, N):
+    """N = 200, 1500, 2100, 3500: 1, 2, 4 and 8 particles per thread, each
+    with a partly filled last wavefront (the tape's padded layout)."""
     import torch
 
     from frictionqpotspringblock_amd import adjoint as A
 
     rng = np.random.default_rng(1)
-    R, N, n, K = 3, 200, 260, 64  # last window partial
+    R, n, K = 3, 260, 64  # last window partial
     b = _batch(R, N)
     u0, v0, a0, par, y = _setup(b, rng, vscale=1.0)
     uf0 = torch.tensor(b.u_frame)
