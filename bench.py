@@ -42,8 +42,10 @@ One bench step = 100 timeSteps (5.4e7 particle-steps); the default --steps
 
 Prints one JSON line: whole-job particle-steps/s of the batched path, the
 plain one-system path beside it, the roofline of the dominant kernel (timed
-with HIP events on the launch stream) and the CPU oracle timed on a bounded
-sample.
+with HIP events on the launch stream) against the published MI355X peak, the
+CPU oracle timed on a bounded sample, and ("peaks_measured", after the timed
+region) the two peaks measured on this GPU: device-to-device copy bandwidth
+and FP64 matrix-product throughput.
 """
 
 import argparse
@@ -787,6 +789,49 @@ def run_flow(args):
     }
 
 
+def measure_peaks():
+    """The HBM and FP64 matrix-core peaks measured on this GPU, run after the
+    timed region: a device-to-device copy of 1 GiB (read plus written bytes
+    per second, best of 10) and a float64 matrix product of two 8192 x 8192
+    matrices (torch.matmul, i.e. the vendor BLAS on the FP64 matrix cores;
+    2 n^3 FLOP, best of 5). Both timed with events on torch's stream."""
+    import torch
+
+    def best_ms(fn, reps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return min(a.elapsed_time(b) for a, b in ev)
+
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    ms = best_ms(lambda: y.copy_(x), 10)
+    copy_gbs = 2 * x.numel() * 8 / (ms * 1e-3) / 1e9
+    del x, y
+    n = 8192
+    g = torch.Generator(device="cuda").manual_seed(0)
+    a = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g)
+    b = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g)
+    c = torch.empty_like(a)
+    for _ in range(2):
+        torch.matmul(a, b, out=c)
+    ms = best_ms(lambda: torch.matmul(a, b, out=c), 5)
+    gemm_tf = 2.0 * n**3 / (ms * 1e-3) / 1e12
+    del a, b, c
+    torch.cuda.empty_cache()
+    return {
+        "hbm_copy_GBs": copy_gbs,
+        "fp64_matmul_TFs": gemm_tf,
+        "published": {"hbm_GBs": 8000.0, "fp64_TFs": 78.6},
+        "how": "1 GiB device copy (read + write bytes), best of 10; float64 torch.matmul 8192^3, best of 5",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -803,6 +848,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the copy / matrix-product peak measurement")
     ap.add_argument(
         "--copy-probe", action="store_true",
         help="only run a plain device copy of 1 GiB three times (calibration of HBM byte counters) and exit",
@@ -835,6 +881,8 @@ def main():
         "quasistatic": run_quasistatic,
     }[args.workload]
     line = run(args)
+    if not args.no_peaks:
+        line["peaks_measured"] = measure_peaks()
     print(json.dumps(line), flush=True)
 
 
