@@ -636,7 +636,7 @@ def rg_tile_roofline(replicas=8, n=65536, steps=2048):
     achieved = FLOP_PER_PARTICLE_STEP * ps / (ms * 1e-3) / 1e12
     return {
         "kernel": f"k_tile_block<Cuspy, Laplace1d> + k_tile_block_decide ({replicas} chains x {n}, "
-        "blocks of 128 steps over tiles of 2048 particles)",
+        "blocks of 128 steps over tiles of 1024 particles)",
         "bound": "valu",
         "achieved": achieved,
         "peak": 78.6,

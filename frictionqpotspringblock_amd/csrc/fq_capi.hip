@@ -1676,7 +1676,7 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
             g->tile_part = dalloc<double>(2 * tl.size());
             g->tile_k = 128;
             if (const char* e = std::getenv("FQSB_TILE_K")) {
-                g->tile_k = std::max(0, std::min(std::atoi(e), 256));
+                g->tile_k = std::max(0, std::min(std::atoi(e), fq::TB_KMAX_STEPS));
             }
             if (g->tile_k > 0 && fq::tile_block_supported(g->pot, inter)) {
                 std::vector<fq::TileDesc> tb;

@@ -161,8 +161,10 @@ struct SysView {
 // Uniform arguments of one resident launch.
 // One tile of the tiled per-step path (fq_tile.hpp): particles [p0, p1) of
 // system `sys` (whole rows for lattices).
-// particles per tile of the blocked tiled path (k_tile_block)
-constexpr int64_t TB_TILE_PARTICLES = 2048;
+// particles per tile of the blocked tiled path (k_tile_block), and the
+// longest block of steps its extended segment allows
+constexpr int64_t TB_TILE_PARTICLES = 1024;
+constexpr int TB_KMAX_STEPS = 128;
 
 struct TileDesc {
     int32_t sys;

@@ -288,9 +288,10 @@ __global__ void __launch_bounds__(64) k_tile_decide(const SysView* __restrict__ 
 // NAN, as k_tile_decide does).
 // ---------------------------------------------------------------------------
 
-constexpr int TB_THREADS = 512;
+constexpr int TB_THREADS = 256;
 constexpr int TB_PPT = 5;                 // particles per thread of the extended segment
-constexpr int TB_KMAX = (TB_THREADS * TB_PPT - TB_TILE_PARTICLES) / 2; // 256
+constexpr int TB_KMAX = (TB_THREADS * TB_PPT - TB_TILE_PARTICLES) / 2; // 128
+static_assert(TB_KMAX == TB_KMAX_STEPS, "extended segment and block length disagree");
 constexpr int32_t TB_NO_FAIL = 0x7fffffff;
 
 // steps the block of system S takes (the redo count, or min(K, target - steps))
