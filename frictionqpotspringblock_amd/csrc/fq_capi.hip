@@ -395,12 +395,13 @@ struct Group {
         // timeSteps / flowSteps of chains: the pipelined k_flow (same results)
         const bool chain = inter == FQSB_INT_NONE || inter == FQSB_INT_LAPLACE1D || inter == FQSB_INT_QUARTIC1D ||
                            inter == FQSB_INT_QUARTICGRADIENT1D;
-        bool unit_mass = true;
+        bool unit_mass = true, full = true;
         for (const auto& S : hviews) {
             unit_mass &= S.inv_m == 1.0;
+            full &= S.N % ppt == 0;
         }
         if (ra.mode == fq::MODE_STEPS && chain && !general_kernel &&
-            fq::launch_flow(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream, unit_mass)) {
+            fq::launch_flow(pot, inter, ext, ppt, maxt, threads, lds, views, (int)R, ra, stream, unit_mass, full)) {
             HIPC(hipGetLastError());
             return;
         }

@@ -12,7 +12,12 @@
 //  * the mid-point of the current well, 0.5 (y_l + y_r), is kept in a
 //    register (recomputed when the well changes) for the Cuspy force;
 //  * with m = 1 the multiplication by 1/m (exact) is skipped;
-//  * no event / residual bookkeeping.
+//  * no event / residual bookkeeping;
+//  * FULL (every system of the launch has N divisible by PPT, so a thread
+//    holds PPT particles or none): the thread's last particle is register
+//    PPT - 1 at compile time, which removes a select per register and step
+//    from the interaction and makes the publication of the last slip one
+//    store.
 #pragma once
 
 #include "fq_resident.hpp"
@@ -32,7 +37,7 @@ FQ_HD double potential_force_mid(const SysView& S, double u, double yl, double y
     }
 }
 
-template <int POT, int INT, int EXT, int PPT, int MAXT, bool UNIT>
+template <int POT, int INT, int EXT, int PPT, int MAXT, bool UNIT, bool FULL = false>
 __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views, RunArgs ra)
 {
     static_assert(is_chain<INT>(), "k_flow handles chains");
@@ -55,6 +60,9 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     const int64_t base = (int64_t)t * PPT;
     const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
     const int Tact = (int)((N + PPT - 1) / PPT);
+    // register of the thread's last particle (FULL: cnt is PPT or 0, and a
+    // thread without particles neither publishes nor stores)
+    const int last = FULL ? PPT - 1 : cnt - 1;
     const Landscape& L = S.ls;
     double* const Y = L.y;
     const int64_t nch = L.nchunk;
@@ -136,7 +144,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
         }
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            if (i == cnt - 1) {
+            if (i == last) {
                 h_last[b * T + t] = x[i];
             }
         }
@@ -184,7 +192,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             for (int q = 0; q < PPT; ++q) {
                 if (q == i) {
                     um = (q == 0) ? left_of(b) : x[q > 0 ? q - 1 : 0];
-                    up = (q == cnt - 1) ? right_of(b) : x[q + 1 < PPT ? q + 1 : q];
+                    up = (q == last) ? right_of(b) : x[q + 1 < PPT ? q + 1 : q];
                 }
             }
             return chain_force<INT>(S, um, x[i], up);
@@ -309,10 +317,10 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             v[i] = vv;
             a[i] = aa;
         }
-        if (cnt < PPT) { // padding registers stay at rest
+        if (cnt < PPT) { // padding registers stay at rest (FULL: all of them)
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                if (i >= cnt) {
+                if (FULL || i >= cnt) {
                     v[i] = 0.0;
                     a[i] = 0.0;
                 }

@@ -368,6 +368,50 @@ def test_launch_shapes_agree(fq):
         assert np.array_equal(b.u, ref), (ppt, threads)
 
 
+@pytest.mark.parametrize("N", [4096, 4032, 4090])
+def test_flow_full_threads(fq, oracle, N):
+    """timeSteps / flowSteps with 4 particles per thread on 1024 threads
+    against the oracle: N = 4096 (every thread full: the compile-time last
+    register), 4032 (full, with 16 threads that hold no particle), 4090 (a
+    partly filled thread: the general form)."""
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R = 2
+    kw = dict(
+        potential="Cuspy",
+        interactions="Laplace1d",
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=0,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=200,
+    )
+    b = Batch(R, seed_stride=1, **kw)
+    b.set_launch(4, 1024)
+    v0 = np.random.default_rng(3).normal(0.0, 2.0, (R, N))
+    b.v = v0
+    b.flowSteps(600, 1e-2)
+    b.timeSteps(100)
+    for r in range(R):
+        o = oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, seed=r, nchunk=200))
+        o.v = v0[r]
+        o.flowSteps(600, 1e-2)
+        o.timeSteps(100)
+        assert np.array_equal(b.u[r], o.u)
+        assert np.array_equal(b.v[r], o.v)
+        assert np.array_equal(b.a[r], o.a)
+        assert np.array_equal(b.f[r], o.f)
+        assert np.array_equal(b.index_at_align[r], o.chunk.index_at_align)
+        assert b.u_frame[r] == o.u_frame
+
+
 def test_errors(fq):
     s = fq.Line1d.System_Cuspy_Laplace(**P.delta_par(5))
     s.u = np.full(5, np.nan)
