@@ -71,7 +71,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 # each scaled by the ratio of the true bytes of a 1 GiB device copy to what
 # the same counter reads for it (reads x 2.000, writes x 1.000);
 # profiles/k_flow_cuspy_laplace_flow_R1024_N4096.md
-FLOW_TRAFFIC_BYTES = (501_500 * 2.000 + 792_000 * 1.000) * 1024
+FLOW_TRAFFIC_BYTES = (502_477 * 2.000 + 1_030_076 * 1.000) * 1024
 
 LR_R = 256
 LR_N = 8192
@@ -731,6 +731,10 @@ def run_flow(args):
     tuned = {}
     if not args.no_autotune:
         tuned = batch.autotune(V_FRAME, steps=BLOCK)
+        # the profiled shape (its HBM counters are in FLOW_TRAFFIC_BYTES)
+        # unless another one is clearly faster: trial times within 2% are noise
+        if (4, 1024) in tuned and tuned[(4, 1024)] <= 1.02 * min(tuned.values()):
+            batch.set_launch(4, 1024)
     for _ in range(args.warmup):
         batch.flowSteps(BLOCK, V_FRAME, block=BLOCK)
     torch.cuda.synchronize()
