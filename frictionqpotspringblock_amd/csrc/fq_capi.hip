@@ -1332,10 +1332,14 @@ struct Group {
             ra.max_steps = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), (int64_t)MIN_BLOCK));
         }
         const size_t lds = fq::resident_lds(inter, N, threads);
+        bool full = true;
+        for (const auto& S : hviews) {
+            full &= S.N % ppt == 0;
+        }
         push();
         HIPC(hipEventRecord(ev0, stream));
         for (bool first_launch = true;; first_launch = false) {
-            if (!fq::launch_quasi(pot, inter, ppt, maxt, threads, lds, views, (int)R, ra, qa, stream)) {
+            if (!fq::launch_quasi(pot, inter, ppt, maxt, threads, lds, views, (int)R, ra, qa, stream, full)) {
                 if (first_launch) {
                     return false;
                 }

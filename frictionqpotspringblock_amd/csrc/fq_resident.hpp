@@ -223,7 +223,7 @@ __device__ __noinline__ void refill_wave(const SysView* __restrict__ views, int6
 // step budget, a NaN, max_iter or the end of the run.
 // A step (or displacement) that needs a landscape refill writes the state
 // back, refills (refill_wave) and re-enters with the state reloaded.
-template <int POT, int INT, int EXT, int PPT, int MAXT, bool QS>
+template <int POT, int INT, int EXT, int PPT, int MAXT, bool QS, bool FULL = false>
 __device__ __forceinline__ void resident_body(const SysView* __restrict__ views, const RunArgs& ra, const QsArgs& qa)
 {
     constexpr bool CHAIN = is_chain<INT>();
@@ -247,6 +247,11 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
     const int64_t N = S.N;
     const int64_t base = (int64_t)t * PPT;
     const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
+    // FULL (every system of the launch has N divisible by PPT): a thread
+    // holds PPT particles or none, so whether register i holds a particle is
+    // one test per thread, and the last particle is register PPT - 1
+    const int last = FULL ? PPT - 1 : cnt - 1;
+    auto own = [&](int i) -> bool { return FULL ? cnt > 0 : i < cnt; };
     const int Tact = (int)((N + PPT - 1) / PPT);
     const Landscape& L = S.ls;
     const bool track = ra.mode != MODE_STEPS;
@@ -308,7 +313,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
             u[i] = v[i] = a[i] = yl[i] = yr[i] = fth[i] = 0.0;
             jl[i] = 0;
             dl[i] = 0;
-            if (i < cnt) {
+            if (own(i)) {
                 const int64_t p = base + i;
                 u[i] = S.u[p];
                 v[i] = S.v[p];
@@ -341,12 +346,12 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 if (cnt > 0) {
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
-                        if (i == 0 || i == cnt - 1) {
+                        if (i == 0 || i == last) {
                             const double x = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
                             if (i == 0) {
                                 h_first[b * T + t] = x;
                             }
-                            if (i == cnt - 1) {
+                            if (i == last) {
                                 h_last[b * T + t] = x;
                             }
                         }
@@ -356,7 +361,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
             else {
 #pragma unroll
                 for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
+                    if (own(i)) {
                         mir[(int64_t)b * N + base + i] = next ? u[i] + dt * v[i] + c2 * a[i] : u[i];
                     }
                 }
@@ -381,7 +386,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
             long long mn = 0x7fffffffffffffffLL;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                if (i < cnt) {
+                if (own(i)) {
                     long long nx = S.next[base + i];
                     mn = nx < mn ? nx : mn;
                 }
@@ -404,7 +409,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 for (int q = 0; q < PPT; ++q) {
                     if (q == i) {
                         um = (q == 0) ? left_of(b) : u[q > 0 ? q - 1 : 0];
-                        up = (q == cnt - 1) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
+                        up = (q == last) ? right_of(b) : u[q + 1 < PPT ? q + 1 : q];
                     }
                 }
                 return chain_force<INT>(S, um, u[i], up);
@@ -473,7 +478,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
             uint32_t need = 0;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                if (i < cnt) {
+                if (own(i)) {
                     u[i] = u[i] + du;
                     const double x = u[i];
                     if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
@@ -521,7 +526,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                     double sfr = 0.0;
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt) {
+                        if (own(i)) {
                             sd += dl[i];
                             sfr += kf * (uf - u[i]);
                         }
@@ -555,7 +560,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                     long long sd = 0;
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt) {
+                        if (own(i)) {
                             sd += dl[i];
                         }
                     }
@@ -579,7 +584,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                         int outside = 0;
 #pragma unroll
                         for (int i = 0; i < PPT; ++i) {
-                            if (i < cnt) {
+                            if (own(i)) {
                                 const double x =
                                     max_uniform_particle<POT>(S, u[i], yl[i], yr[i], qa.direction, outside);
                                 m = x < m ? x : m;
@@ -636,7 +641,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                     uint32_t refill = 0; // particles whose next well is outside the safe zone
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
-                        if (i < cnt) {
+                        if (own(i)) {
                             const double x = u[i] + dt * v[i] + c2 * a[i];
                             if (!(yl[i] <= x && x < yr[i]) && __builtin_isfinite(x)) {
                                 if (!reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], x, jmin, jmax)) {
@@ -733,7 +738,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                         int due = 0;
 #pragma unroll
                         for (int i = 0; i < PPT; ++i) {
-                            if (i < cnt && inc >= S.next[base + i]) {
+                            if (own(i) && inc >= S.next[base + i]) {
                                 ++due;
                             }
                         }
@@ -743,7 +748,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                             uint64_t st = pcg_advance(rng, rng_inc, 2 * (int64_t)rank);
 #pragma unroll
                             for (int i = 0; i < PPT; ++i) {
-                                if (i < cnt) {
+                                if (own(i)) {
                                     const int64_t p = base + i;
                                     int64_t nx = S.next[p];
                                     if (inc >= nx) {
@@ -756,7 +761,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                         long long mn = 0x7fffffffffffffffLL;
 #pragma unroll
                         for (int i = 0; i < PPT; ++i) {
-                            if (i < cnt) {
+                            if (own(i)) {
                                 long long nx = S.next[base + i];
                                 mn = nx < mn ? nx : mn;
                             }
@@ -771,7 +776,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 int nanf = 0;
 #pragma unroll
                 for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
+                    if (own(i)) {
                         u[i] = u[i] + dt * v[i] + c2 * a[i];
                         const double x = u[i];
                         nanf |= __builtin_isnan(x);
@@ -806,7 +811,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
                 int nchg = 0, sabs = 0;
 #pragma unroll
                 for (int i = 0; i < PPT; ++i) {
-                    if (i < cnt) {
+                    if (own(i)) {
                         const double fi = interaction(i, b);
                         const double fp = potential_force<POT>(S, u[i], yl[i], yr[i]);
                         const double ff = kf * (uf - u[i]);
@@ -904,7 +909,7 @@ __device__ __forceinline__ void resident_body(const SysView* __restrict__ views,
         }
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
+            if (own(i)) {
                 const int64_t p = base + i;
                 S.u[p] = u[i];
                 S.v[p] = v[i];
@@ -975,11 +980,12 @@ __global__ void __launch_bounds__(MAXT) k_resident(const SysView* __restrict__ v
 
 // Quasistatic driver: the events of QsArgs for every system, one workgroup
 // per system, relaunched when its step budget runs out.
-template <int POT, int INT, int PPT, int MAXT>
+// FULL: every system of the launch has N divisible by PPT.
+template <int POT, int INT, int PPT, int MAXT, bool FULL = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT <= 256 && PPT <= 8 ? 2 : 1)))
 k_quasi(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 {
-    resident_body<POT, INT, 0, PPT, MAXT, true>(views, ra, qa);
+    resident_body<POT, INT, 0, PPT, MAXT, true, FULL>(views, ra, qa);
 }
 
 } // namespace fq
