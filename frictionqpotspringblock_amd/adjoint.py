@@ -11,7 +11,11 @@ k_interactions, k_frame, eta) and y [R, N, nchunk]. Well indices are held at
 the values the trajectory used and are not differentiated.
 
 Cuspy + Laplace1d replicas (``Batch(..., potential="Cuspy",
-interactions="Laplace1d")``), N <= 4096, no external forcing.
+interactions="Laplace1d")``), N <= 4096, no external forcing. A batch holds
+the tape of its last forward pass only: the backward pass of an earlier
+``run`` on the same batch (or one after any other call that changed the
+batch) raises instead of using another trajectory; chain runs on separate
+batches.
 """
 
 from __future__ import annotations
@@ -55,6 +59,15 @@ def chunk_data(batch, device="cuda"):
     return out
 
 
+def tape_generation(batch):
+    """Which tape the batch holds: increases with every forward pass of
+    `run`; 0 once the batch's state, landscape or parameters changed after
+    the last one (fqsb_batch_adjoint_generation)."""
+    g = C.c_uint64()
+    _check(capi().fqsb_batch_adjoint_generation(batch._h, C.byref(g)))
+    return g.value
+
+
 def get_field(batch, name, device="cuda"):
     out = torch.empty((batch.replicas, batch.size), dtype=torch.float64, device=device)
     torch.cuda.current_stream(out.device).synchronize()
@@ -86,6 +99,7 @@ class _VerletAdjoint(torch.autograd.Function):
             _check(lib.fqsb_batch_get_device(h, FIELD[name], _ptr(t)))
             outs.append(t)
         ctx.batch = batch
+        ctx.generation = tape_generation(batch)
         ctx.has_y = y is not None
         ctx.y_shape = None if y is None else tuple(y.shape)
         ctx.par_device = torch.as_tensor(params).device
@@ -97,6 +111,14 @@ class _VerletAdjoint(torch.autograd.Function):
     def backward(ctx, gu, gv, ga):
         batch, dev = ctx.batch, ctx.dev
         R, N = batch.replicas, batch.size
+        # the batch holds one tape (the last forward pass), recomputed from its
+        # current landscape and parameters: refuse a backward pass whose
+        # forward pass is no longer the one the batch holds
+        if tape_generation(batch) != ctx.generation:
+            raise RuntimeError(
+                "the adjoint tape of this forward pass is gone: the batch ran another forward pass or changed "
+                "its state, landscape or parameters since; a batch differentiates its last run() only"
+            )
         g = []
         for x in (gu, gv, ga):
             g.append(torch.zeros((R, N), dtype=torch.float64, device=dev) if x is None else _f64(x, dev))

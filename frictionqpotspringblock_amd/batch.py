@@ -209,6 +209,7 @@ SIGNATURES = {
     "fqsb_batch_adjoint_forward": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int, C.c_double]),
     "fqsb_batch_adjoint_backward_device": (C.c_int, [_P] + [C.c_void_p] * 8),
     "fqsb_batch_adjoint_kernel_ms": (C.c_int, [_P, _fp, _fp]),
+    "fqsb_batch_adjoint_generation": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "fqsb_ragged_create": (C.c_int, [C.POINTER(Desc), C.c_uint64, C.POINTER(_P)]),
     "fqsb_ragged_destroy": (C.c_int, [_P]),
     "fqsb_ragged_systems": (C.c_uint64, [_P]),
@@ -279,6 +280,35 @@ def _quasistatic(call, nsys, eps, nevents, first_event, direction, kicks_only, t
     ]
     _check(call(*args))
     return out
+
+
+def validate_checkpoint(ck, replicas, shape, nchunk, external):
+    """Check a checkpoint (Batch.checkpoint, or an .npz written by
+    Batch.save) against a batch of `replicas` systems of `shape` with windows
+    of `nchunk` yield positions: every item present with its size, and every
+    local well index inside the window (the device reads y[p][jloc] and
+    y[p][jloc + 1]). Raises RuntimeError; runs on the host only."""
+    n = replicas * int(np.prod(shape))
+    sizes = {
+        "u": n, "v": n, "a": n, "y": n * nchunk, "start": n, "jloc": n, "st_begin": n, "st_end": n,
+        "inc": replicas, "u_frame": replicas, "qs_first": replicas, "qs_last": replicas,
+    }
+    if external:
+        sizes.update(f_thermal=n, next=n, rng_state=replicas)
+    for k, size in sizes.items():
+        if k not in ck:
+            raise RuntimeError(f"checkpoint item {k} is missing")
+        if np.size(ck[k]) != size:
+            raise RuntimeError(f"checkpoint item {k} has {np.size(ck[k])} values, the batch needs {size}")
+    jl = np.asarray(ck["jloc"]).ravel()
+    bad = np.flatnonzero((jl < 0) | (jl > nchunk - 2))
+    if bad.size:
+        raise RuntimeError(
+            f"checkpoint: local well index {int(jl[bad[0]])} of particle {int(bad[0])} is outside the window "
+            f"[0, {nchunk - 2}]"
+        )
+    if not np.all(np.isfinite(np.asarray(ck["y"]))):
+        raise RuntimeError("checkpoint: yield positions are not finite")
 
 
 def save_events(path, q, system=None):
@@ -401,6 +431,11 @@ class Batch:
     @property
     def shape(self):
         return list(self._shape)
+
+    @property
+    def nchunk(self):
+        """Yield positions held per particle (the window of chunk.data)."""
+        return int(self._desc.nchunk)
 
     @property
     def stream(self):
@@ -583,7 +618,10 @@ class Batch:
         return ck
 
     def restore(self, ck):
-        """Continue from a checkpoint of a batch of the same description."""
+        """Continue from a checkpoint of a batch of the same description. The
+        whole checkpoint is checked first (validate_checkpoint), so a bad one
+        raises without changing the batch."""
+        validate_checkpoint(ck, self._R, self._shape, self.nchunk, bool(self._desc.external))
         for k in ("y", "start", "jloc", "st_begin", "st_end", "inc", "u_frame", "qs_first", "qs_last"):
             self._set_state(k, ck[k])
         if self._desc.external:

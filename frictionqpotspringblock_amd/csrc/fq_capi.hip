@@ -117,6 +117,7 @@ struct Group {
     int adj_flow = 0;
     double adj_vf = 0.0;
     bool adj_valid = false;
+    uint64_t adj_gen = 0; // number of adjoint forward passes: names the tape a backward pass may use
     double *ck_u = nullptr, *ck_v = nullptr, *ck_a = nullptr;
     int32_t* ck_j = nullptr;
     std::vector<double> ck_uf; // [nw][R]
@@ -724,6 +725,7 @@ struct Group {
         adj_flow = flow ? 1 : 0;
         adj_vf = v_frame;
         adj_valid = true;
+        ++adj_gen;
     }
 
     // cotangents of (u, v, a) after the last adjoint_forward -> cotangents of
@@ -2513,6 +2515,7 @@ int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads)
 int fqsb_batch_flow_steps(fqsb_batch* b, uint64_t n, double v_frame, uint64_t block, float* ms)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         float t = b->g->steps(n, true, v_frame, block, false, ms != nullptr);
         if (ms) {
             *ms = t;
@@ -2523,6 +2526,7 @@ int fqsb_batch_flow_steps(fqsb_batch* b, uint64_t n, double v_frame, uint64_t bl
 int fqsb_batch_time_steps(fqsb_batch* b, uint64_t n, uint64_t block, float* ms)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         float t = b->g->steps(n, false, 0.0, block, false, ms != nullptr);
         if (ms) {
             *ms = t;
@@ -2534,6 +2538,7 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
                         uint64_t* ret)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         Group& g = *b->g;
         if (g.minim != FQSB_MIN_INERTIAL) {
             throw Error("batched minimise needs inertial systems");
@@ -2565,6 +2570,7 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
 int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direction, double* ret)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         Group& g = *b->g;
         if (g.ext) {
             throw Error("eventDrivenStep is not available for this system");
@@ -2579,6 +2585,7 @@ int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, 
                            double* f_frame, int64_t* inc)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         b->g->quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame,
                           inc);
     });
@@ -2597,18 +2604,25 @@ int fqsb_batch_minimise_truncate(fqsb_batch* b, const int64_t* i_n, uint64_t A_t
                                  uint64_t* ret)
 {
     return guard([&] {
+        b->g->adj_valid = false;
         b->g->minimise_truncate(i_n, A_truncate, S_truncate, tol, niter_tol, max_iter, max_iter_is_error != 0, ret);
     });
 }
 
 int fqsb_batch_trigger(fqsb_batch* b, const uint64_t* p, double eps, int direction)
 {
-    return guard([&] { b->g->trigger_each(p, eps, direction); });
+    return guard([&] {
+        b->g->adj_valid = false;
+        b->g->trigger_each(p, eps, direction);
+    });
 }
 
 int fqsb_batch_advance_to_fixed_force(fqsb_batch* b, const double* f_frame, int allow_plastic)
 {
-    return guard([&] { b->g->advance_to_fixed_force(f_frame, allow_plastic != 0); });
+    return guard([&] {
+        b->g->adj_valid = false;
+        b->g->advance_to_fixed_force(f_frame, allow_plastic != 0);
+    });
 }
 
 
@@ -2641,6 +2655,20 @@ int fqsb_batch_set_state(fqsb_batch* b, int what, const void* in)
     return guard([&] {
         Group& g = *b->g;
         const StateItem it = state_item(g, what);
+        g.adj_valid = false;
+        if (what == FQSB_STATE_JLOC) {
+            // every kernel reads y[p][jloc] and y[p][jloc + 1] of the window:
+            // a checkpoint of another nchunk (or corrupt data) is refused
+            // here, before anything reaches the device
+            const int32_t* jl = (const int32_t*)in;
+            const int64_t hi = (int64_t)g.nchunk - 2;
+            for (size_t i = 0; i < it.count; ++i) {
+                if (jl[i] < 0 || (int64_t)jl[i] > hi) {
+                    throw Error("checkpoint: local well index " + std::to_string(jl[i]) + " of particle " +
+                                std::to_string(i) + " is outside the window [0, " + std::to_string(hi) + "]");
+                }
+            }
+        }
         if (it.dev) {
             g.h2d((char*)it.dev, (const char*)in, it.count * (size_t)it.bytes);
         }
@@ -2666,7 +2694,10 @@ int fqsb_batch_set_state(fqsb_batch* b, int what, const void* in)
 
 int fqsb_batch_refresh(fqsb_batch* b)
 {
-    return guard([&] { b->g->refresh(); });
+    return guard([&] {
+        b->g->adj_valid = false;
+        b->g->refresh();
+    });
 }
 
 int fqsb_batch_get(fqsb_batch* b, int field, double* out)
@@ -2682,6 +2713,7 @@ int fqsb_batch_set(fqsb_batch* b, int field, const double* in)
     return guard([&] {
         Group& g = *b->g;
         const size_t n = (size_t)g.total;
+        g.adj_valid = false;
         switch (field) {
             case FQSB_U:
                 g.h2d(g.u, in, n);
@@ -2739,7 +2771,10 @@ void* fqsb_batch_stream(fqsb_batch* b)
 
 int fqsb_batch_set_parameters(fqsb_batch* b, const double* params)
 {
-    return guard([&] { b->g->set_parameters(params); });
+    return guard([&] {
+        b->g->adj_valid = false;
+        b->g->set_parameters(params);
+    });
 }
 
 int fqsb_batch_get_parameters(const fqsb_batch* b, double* params)
@@ -2821,6 +2856,12 @@ int fqsb_batch_adjoint_kernel_ms(const fqsb_batch* b, float* record_ms, float* r
 int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame)
 {
     return guard([&] { b->g->adjoint_forward(n, window, flow != 0, v_frame); });
+}
+
+int fqsb_batch_adjoint_generation(const fqsb_batch* b, uint64_t* gen)
+{
+    *gen = b->g->adj_valid ? b->g->adj_gen : 0;
+    return 0;
 }
 
 int fqsb_batch_adjoint_backward_device(fqsb_batch* b, const double* gu, const double* gv, const double* ga,

@@ -319,7 +319,20 @@ FQ_HD double u32_to_double(uint32_t x)
 }
 
 // ---------------------------------------------------------------------------
-// Elementary functions (fdlibm 5.3 methods, see the oracle for the pins)
+// Elementary functions (fdlibm 5.3 methods, see the oracle for the pins).
+// log, exp, sin, cos and the argument reduction below follow fdlibm 5.3
+// (e_log.c, e_exp.c, k_sin.c, k_cos.c, e_rem_pio2.c), whose notice is:
+//
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+// Developed at SunPro, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this
+// software is freely granted, provided that this notice
+// is preserved.
+//
+// The reduction always runs the second Cody-Waite round (fdlibm skips it
+// when the first one is already accurate), so fq_sin / fq_cos can differ
+// from fdlibm in the last bit; the CPU oracle does the same.
 // ---------------------------------------------------------------------------
 
 FQ_HD uint32_t hiword(double x)

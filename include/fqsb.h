@@ -338,6 +338,12 @@ int fqsb_batch_set_chunk_data_device(fqsb_batch* b, const double* in);
 int fqsb_batch_adjoint_forward(fqsb_batch* b, uint64_t n, uint64_t window, int flow, double v_frame);
 int fqsb_batch_adjoint_backward_device(fqsb_batch* b, const double* gu, const double* gv, const double* ga,
                                        double* gu0, double* gv0, double* ga0, double* gparams, double* gy);
+/* Which tape the batch holds: a number that increases with every
+ * fqsb_batch_adjoint_forward, or 0 once any later call has changed the
+ * batch's state, landscape or parameters (a backward pass then has no valid
+ * tape). A caller that keeps the number of its forward pass can tell that its
+ * tape was replaced by a later forward pass on the same batch. */
+int fqsb_batch_adjoint_generation(const fqsb_batch* b, uint64_t* gen);
 /* Average duration (ms) of one window's recompute (record) and reverse
  * kernel in the last backward pass (HIP events on the batch's stream). */
 int fqsb_batch_adjoint_kernel_ms(const fqsb_batch* b, float* record_ms, float* reverse_ms);

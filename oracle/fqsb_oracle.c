@@ -85,6 +85,21 @@ const char* fqo_last_error(void)
 /* ------------------------------------------------------------------------- */
 /* fdlibm-method elementary functions                                        */
 /* ------------------------------------------------------------------------- */
+/*
+ * log, exp, sin, cos and the argument reduction below follow fdlibm 5.3
+ * (e_log.c, e_exp.c, k_sin.c, k_cos.c, e_rem_pio2.c), whose notice is:
+ *
+ * Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+ *
+ * Developed at SunPro, a Sun Microsystems, Inc. business.
+ * Permission to use, copy, modify, and distribute this
+ * software is freely granted, provided that this notice
+ * is preserved.
+ *
+ * rem_pio2 always runs the second Cody-Waite round (fdlibm skips it when the
+ * first is already accurate), so sin/cos may differ from fdlibm in the last
+ * bit; the device code does the same.
+ */
 
 typedef union {
     double d;
@@ -515,6 +530,19 @@ static inline double draw_normal(uint64_t* state, uint64_t inc, double mean, dou
     return mean + stddev * z;
 }
 
+int fqo_draw_normal(uint64_t* state, uint64_t inc, double mean, double stddev, double* out, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        out[i] = draw_normal(state, inc, mean, stddev);
+    }
+    return 0;
+}
+
+uint64_t fqo_pcg32_default_stream(void)
+{
+    return PCG_DEFAULT_STREAM;
+}
+
 /* ------------------------------------------------------------------------- */
 /* chunked landscape                                                         */
 /* ------------------------------------------------------------------------- */
@@ -722,6 +750,9 @@ struct fqo_system {
     uint64_t rng_state, rng_inc;
     int64_t *next, *dinc;
     int has_ext;
+    /* smallest relative change of one residual that would change the step at
+     * which the last minimise-type call stopped (see stoplist_margin) */
+    double stop_margin;
 };
 
 uint64_t fqo_size(const fqo_system* s)
@@ -1026,6 +1057,7 @@ int fqo_create(const fqo_desc* d, fqo_system** out)
     if (!s) {
         return fail("out of memory");
     }
+    s->stop_margin = INFINITY;
     s->d = *d;
     s->N = N;
     s->rows = rows;
@@ -1348,6 +1380,99 @@ static int stoplist_stop(const stoplist* l, double tol, double tol2)
     return (stoplist_descending(l) && stoplist_all_less(l, tol)) || stoplist_all_less(l, tol2);
 }
 
+/* Relative distance of a from the threshold b (infinite when either is not
+ * finite: a comparison with the ring's initial +inf cannot flip). */
+static inline double rel_gap(double a, double b)
+{
+    if (!isfinite(a) || !isfinite(b) || b == 0.0) {
+        return INFINITY;
+    }
+    return fabs(a - b) / fabs(b);
+}
+
+/* How robust the StopList's decision at this step is: the smallest relative
+ * perturbation of the residuals in the ring that would flip it. Not stopped:
+ * each branch ((descending && all_less(tol)) or all_less(tol^2)) becomes true
+ * only if every failing comparison of it flips, so the branch needs the
+ * largest gap of its failing comparisons; the decision flips at the smaller of
+ * the two branches. Stopped: every true branch must become false, each by its
+ * smallest gap. Not a check of the reference: it measures how far the
+ * residuals were from a tie, so that a residual computed in another summation
+ * order (a few ulp away) is known to give the same stop step. */
+static double stoplist_margin(const stoplist* l, double tol, double tol2, int stopped)
+{
+    const uint64_t n = l->n;
+    if (n == 0) {
+        return INFINITY;
+    }
+    double fail_ad = 0.0, fail_b = 0.0; /* largest gap of a failing comparison */
+    double min_ad = INFINITY, min_b = INFINITY; /* smallest gap of any comparison */
+    int ad = 1, b = 1;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double x = l->r[i];
+        double g = rel_gap(x, tol);
+        min_ad = g < min_ad ? g : min_ad;
+        if (!(x < tol)) {
+            ad = 0;
+            fail_ad = g > fail_ad ? g : fail_ad;
+        }
+        g = rel_gap(x, tol2);
+        min_b = g < min_b ? g : min_b;
+        if (!(x < tol2)) {
+            b = 0;
+            fail_b = g > fail_b ? g : fail_b;
+        }
+        if (i > 0) {
+            g = rel_gap(x, l->r[i - 1]);
+            min_ad = g < min_ad ? g : min_ad;
+            if (!(x < l->r[i - 1])) {
+                ad = 0;
+                fail_ad = g > fail_ad ? g : fail_ad;
+            }
+        }
+    }
+    if (!stopped) {
+        const double make_ad = ad ? 0.0 : fail_ad, make_b = b ? 0.0 : fail_b;
+        return make_ad < make_b ? make_ad : make_b;
+    }
+    const double break_ad = ad ? min_ad : 0.0, break_b = b ? min_b : 0.0;
+    return break_ad > break_b ? break_ad : break_b;
+}
+
+static inline void track_margin(fqo_system* s, const stoplist* l, double tol, double tol2, int stopped)
+{
+    const double m = stoplist_margin(l, tol, tol2, stopped);
+    if (m < s->stop_margin) {
+        s->stop_margin = m;
+    }
+}
+
+double fqo_last_stop_margin(const fqo_system* s)
+{
+    return s->stop_margin;
+}
+
+int fqo_stoplist_margin(const double* residuals, uint64_t n, uint64_t niter_tol, double tol, double* margin)
+{
+    stoplist l;
+    if (stoplist_init(&l, niter_tol)) {
+        return 1;
+    }
+    double m = INFINITY;
+    for (uint64_t i = 0; i < n; ++i) {
+        stoplist_roll_insert(&l, residuals[i]);
+        const int stop = stoplist_stop(&l, tol, tol * tol);
+        const double g = stoplist_margin(&l, tol, tol * tol, stop);
+        m = g < m ? g : m;
+        if (stop) {
+            break;
+        }
+    }
+    *margin = m;
+    free(l.r);
+    return 0;
+}
+
 int fqo_stoplist_run(const double* residuals, uint64_t n, uint64_t niter_tol, double tol, int64_t* stop_index)
 {
     stoplist l;
@@ -1382,6 +1507,7 @@ int fqo_time_steps_until_event(fqo_system* s, double tol, uint64_t niter_tol, ui
     if (stoplist_init(&res, niter_tol)) {
         return 1;
     }
+    s->stop_margin = INFINITY;
     int64_t* i_n = (int64_t*)malloc(sizeof(int64_t) * s->N);
     for (uint64_t p = 0; p < s->N; ++p) {
         i_n[p] = gidx(s, p);
@@ -1404,6 +1530,7 @@ int fqo_time_steps_until_event(fqo_system* s, double tol, uint64_t niter_tol, ui
             break;
         }
         stoplist_roll_insert(&res, fqo_residual(s));
+        track_margin(s, &res, tol, tol2, stoplist_stop(&res, tol, tol2));
         if (stoplist_stop(&res, tol, tol2)) {
             fqo_quench(s);
             step = 0;
@@ -1427,6 +1554,7 @@ static int minimise_nopassing(fqo_system* s, double tol, uint64_t niter_tol, uin
     if (stoplist_init(&res, niter_tol)) {
         return 1;
     }
+    s->stop_margin = INFINITY;
     s->qs_first = s->inc;
     s->qs_last = s->inc;
     const double k = s->k;
@@ -1466,6 +1594,7 @@ static int minimise_nopassing(fqo_system* s, double tol, uint64_t niter_tol, uin
             s->f[p] = s->f_pot[p] + s->f_int[p] + s->f_frame[p];
         }
         stoplist_roll_insert(&res, fqo_residual(s));
+        track_margin(s, &res, tol, tol2, stoplist_stop(&res, tol, tol2));
         if (stoplist_stop(&res, tol, tol2)) {
             fqo_quench(s);
             free(res.r);
@@ -1499,6 +1628,7 @@ int fqo_minimise(fqo_system* s, double tol, uint64_t niter_tol, uint64_t max_ite
     if (stoplist_init(&res, niter_tol)) {
         return 1;
     }
+    s->stop_margin = INFINITY;
     int64_t* i_n = NULL;
     int64_t sa = 0, s_n = 0;
     int init = 1;
@@ -1531,6 +1661,7 @@ int fqo_minimise(fqo_system* s, double tol, uint64_t niter_tol, uint64_t max_ite
             }
             s_n = sa;
         }
+        track_margin(s, &res, tol, tol2, stoplist_stop(&res, tol, tol2));
         if (stoplist_stop(&res, tol, tol2)) {
             fqo_quench(s);
             free(i_n);
@@ -1562,6 +1693,7 @@ int fqo_minimise_truncate(fqo_system* s, const int64_t* i_n, uint64_t A_truncate
     if (stoplist_init(&res, niter_tol)) {
         return 1;
     }
+    s->stop_margin = INFINITY;
     int64_t sa = 0, s_n = 0;
     int init = 1;
     uint64_t step;
@@ -1586,6 +1718,7 @@ int fqo_minimise_truncate(fqo_system* s, const int64_t* i_n, uint64_t A_truncate
             s->qs_last = s->inc;
         }
         s_n = sa;
+        track_margin(s, &res, tol, tol2, stoplist_stop(&res, tol, tol2));
         if (stoplist_stop(&res, tol, tol2)) {
             fqo_quench(s);
             free(res.r);

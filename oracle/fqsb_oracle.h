@@ -149,6 +149,11 @@ uint64_t fqo_pcg32_advance(uint64_t state, uint64_t inc, int64_t delta);
 double fqo_uint32_to_double(uint32_t x);
 int fqo_draw(int distribution, const double* parameters, int nparameters, uint64_t* state, uint64_t inc, double* out,
              uint64_t n);
+/* The thermal draw of RandomNormalForcing: n consecutive normals of one
+ * pcg32 stream (two uniforms each); the stream a forcing seed s starts from
+ * is fqo_pcg32_seed(s, fqo_pcg32_default_stream()). */
+int fqo_draw_normal(uint64_t* state, uint64_t inc, double mean, double stddev, double* out, uint64_t n);
+uint64_t fqo_pcg32_default_stream(void);
 double fqo_log(double x);
 double fqo_exp(double x);
 double fqo_sin(double x);
@@ -157,6 +162,12 @@ double fqo_pow(double x, double y);
 
 /* StopList restatement (GooseFEM Iterate::StopList) exposed for pinning. */
 int fqo_stoplist_run(const double* residuals, uint64_t n, uint64_t niter_tol, double tol, int64_t* stop_index);
+/* Smallest relative change of a single residual that would change the step at
+ * which the StopList stops on this sequence (the decision margin), and the
+ * same for the last minimise / minimise_truncate / timeStepsUntilEvent call of
+ * a system (+inf if no step came near a decision). */
+int fqo_stoplist_margin(const double* residuals, uint64_t n, uint64_t niter_tol, double tol, double* margin);
+double fqo_last_stop_margin(const fqo_system* s);
 
 #ifdef __cplusplus
 }

@@ -158,12 +158,16 @@ def lib():
         "fqo_pcg32_advance": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int64]),
         "fqo_uint32_to_double": (C.c_double, [C.c_uint32]),
         "fqo_draw": (C.c_int, [C.c_int, dp, C.c_int, up, C.c_uint64, dp, C.c_uint64]),
+        "fqo_draw_normal": (C.c_int, [up, C.c_uint64, C.c_double, C.c_double, dp, C.c_uint64]),
+        "fqo_pcg32_default_stream": (C.c_uint64, []),
         "fqo_log": (C.c_double, [C.c_double]),
         "fqo_exp": (C.c_double, [C.c_double]),
         "fqo_sin": (C.c_double, [C.c_double]),
         "fqo_cos": (C.c_double, [C.c_double]),
         "fqo_pow": (C.c_double, [C.c_double, C.c_double]),
         "fqo_stoplist_run": (C.c_int, [dp, C.c_uint64, C.c_uint64, C.c_double, ip]),
+        "fqo_stoplist_margin": (C.c_int, [dp, C.c_uint64, C.c_uint64, C.c_double, dp]),
+        "fqo_last_stop_margin": (C.c_double, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -223,12 +227,47 @@ def draw(distribution, parameters, state, inc, n):
     return out, st.value
 
 
+def draw_normal(state, inc, mean, stddev, n):
+    """n consecutive thermal draws of one pcg32 stream (the draw of
+    RandomNormalForcing::force, detail.h:935-940)."""
+    st = C.c_uint64(state)
+    out = np.empty(n)
+    _check(lib().fqo_draw_normal(C.byref(st), inc, mean, stddev, _dp(out), n))
+    return out, st.value
+
+
+def forcing_stream(seed):
+    """State and increment of the forcing generator of seed `seed`
+    (prrng::pcg32::seed(seed) with the default stream, detail.h:917)."""
+    return pcg32_seed(seed, lib().fqo_pcg32_default_stream())
+
+
 def vec_math(name, x, y=None):
     fn = getattr(lib(), "fqo_" + name)
     x = np.asarray(x, dtype=float).ravel()
     if y is None:
         return np.array([fn(float(v)) for v in x])
     return np.array([fn(float(v), float(y)) for v in x])
+
+
+def stoplist_margin(residuals, niter_tol, tol):
+    """Decision margin of the StopList on a residual sequence: the smallest
+    relative change of one residual that would change its stop step."""
+    r = np.ascontiguousarray(residuals, dtype=float)
+    m = C.c_double()
+    _check(lib().fqo_stoplist_margin(_dp(r), r.size, niter_tol, tol, C.byref(m)))
+    return m.value
+
+
+# (system size, decision margin) of every minimise-type call of any system
+# since the last clear; the GPU reduces the residual norms in another order
+# than the oracle's sequential sum, so its stop step is the oracle's whenever
+# the margin exceeds the rounding bound of the two sums.
+STOP_MARGINS = []
+
+
+def _record_margin(system):
+    STOP_MARGINS.append((system.size, float(lib().fqo_last_stop_margin(system._h))))
 
 
 def stoplist_run(residuals, niter_tol, tol):
@@ -479,6 +518,7 @@ class _System:
                 self._h, tol, niter_tol, int(max_iter), int(time_activity), int(max_iter_is_error), C.byref(r)
             )
         )
+        _record_margin(self)
         return r.value
 
     def minimise_truncate(
@@ -508,6 +548,7 @@ class _System:
                 C.byref(r),
             )
         )
+        _record_margin(self)
         return r.value
 
     def eventDrivenStep(self, eps, kick, direction=1):
@@ -528,6 +569,7 @@ class _System:
     def timeStepsUntilEvent(self, tol=1e-5, niter_tol=10, max_iter=int(1e9)):
         r = C.c_uint64()
         _check(lib().fqo_time_steps_until_event(self._h, tol, niter_tol, int(max_iter), C.byref(r)))
+        _record_margin(self)
         return r.value
 
     def flowSteps(self, n, v_frame):

@@ -85,3 +85,59 @@ def test_api_surface_matches_reference_binding():
     assert not hasattr(fq.Line1d.System_Cuspy_Laplace_RandomForcing, "minimise")
     assert not hasattr(fq.Line1d.System_Cuspy_Laplace_Nopassing, "timeStep")
     assert hasattr(fq.Line1d.System_Cuspy_Laplace_RandomForcing, "external")
+
+
+def _checkpoint(R, N, nchunk, external=False):
+    import numpy as np
+
+    n = R * N
+    ck = {k: np.zeros(n) for k in ("u", "v", "a")}
+    ck.update(
+        y=np.tile(np.arange(nchunk, dtype=float), n),
+        start=np.zeros(n, dtype=np.int64),
+        jloc=np.full(n, nchunk // 2, dtype=np.int32),
+        st_begin=np.zeros(n, dtype=np.uint64),
+        st_end=np.zeros(n, dtype=np.uint64),
+        inc=np.zeros(R, dtype=np.int64),
+        u_frame=np.zeros(R),
+        qs_first=np.zeros(R, dtype=np.int64),
+        qs_last=np.zeros(R, dtype=np.int64),
+    )
+    if external:
+        ck.update(f_thermal=np.zeros(n), next=np.zeros(n, dtype=np.int64), rng_state=np.zeros(R, dtype=np.uint64))
+    return ck
+
+
+@pytest.mark.parametrize("bad", [-1, 99, 4000])
+def test_checkpoint_with_bad_well_index_is_refused_on_the_host(bad):
+    """A checkpoint whose local well index lies outside the window (the device
+    would read y[p][jloc + 1] out of the row) raises before any device call."""
+    from frictionqpotspringblock_amd.batch import validate_checkpoint
+
+    R, N, nchunk = 2, 5, 100
+    ck = _checkpoint(R, N, nchunk)
+    ck["jloc"][3] = nchunk - 2  # the last well of the window is valid
+    validate_checkpoint(ck, R, [N], nchunk, False)
+    ck["jloc"][7] = bad
+    with pytest.raises(RuntimeError, match="local well index .* of particle 7 is outside the window"):
+        validate_checkpoint(ck, R, [N], nchunk, False)
+
+
+def test_checkpoint_items_and_sizes_are_checked_on_the_host():
+    import numpy as np
+
+    from frictionqpotspringblock_amd.batch import validate_checkpoint
+
+    R, N, nchunk = 2, 5, 100
+    ck = _checkpoint(R, N, nchunk)
+    del ck["st_end"]
+    with pytest.raises(RuntimeError, match="st_end is missing"):
+        validate_checkpoint(ck, R, [N], nchunk, False)
+    ck = _checkpoint(R, N, nchunk)
+    with pytest.raises(RuntimeError, match="y has"):
+        validate_checkpoint(ck, R, [N], nchunk + 1, False)
+    with pytest.raises(RuntimeError, match="f_thermal is missing"):
+        validate_checkpoint(ck, R, [N], nchunk, True)
+    ck["y"][3] = np.nan
+    with pytest.raises(RuntimeError, match="not finite"):
+        validate_checkpoint(ck, R, [N], nchunk, False)

@@ -171,3 +171,40 @@ def test_adjoint_errors():
     with pytest.raises(RuntimeError, match="refill"):
         _check(capi().fqsb_batch_adjoint_forward(b._h, 2000, 100, 1, 0.5))
     assert isinstance(Batch, type) and A.parameters(b).shape == (1, 4)
+
+
+def test_backward_of_a_replaced_tape_raises():
+    """A batch holds the tape of its last forward pass: the backward pass of an
+    earlier run on the same batch, or of a run after which the batch was
+    stepped (a refill could have changed the landscape), raises; the last
+    run still differentiates and matches the tangent-linear model."""
+    import torch
+
+    from frictionqpotspringblock_amd import adjoint as A
+
+    rng = np.random.default_rng(3)
+    R, N, n, K = 2, 256, 60, 20
+    b = _batch(R, N)
+    u0, v0, a0, par, y = _setup(b, rng, vscale=0.5)
+    l1 = [x.clone().requires_grad_(True) for x in (u0, v0, a0)]
+    u1, v1, a1 = A.run(b, n, K, *l1, par)
+    g1 = A.tape_generation(b)
+    assert g1 > 0
+    # second run on the same batch, chained on the first one's output
+    u2, _, _ = A.run(b, n, K, u1, v1, a1, par)
+    assert A.tape_generation(b) == g1 + 1
+    with pytest.raises(RuntimeError, match="tape of this forward pass is gone"):
+        u1.sum().backward(retain_graph=True)
+    # the last run differentiates; through it, the first one's backward raises
+    l2 = [x.detach().clone().requires_grad_(True) for x in (u1, v1, a1)]
+    u3, _, _ = A.run(b, n, K, *l2, par)
+    u3.sum().backward()
+    assert all(x.grad is not None and torch.all(torch.isfinite(x.grad)) for x in l2)
+    # stepping the batch after a forward pass invalidates its tape
+    l3 = [x.detach().clone().requires_grad_(True) for x in (u0, v0, a0)]
+    u4, _, _ = A.run(b, n, K, *l3, par)
+    b.timeSteps(5)
+    assert A.tape_generation(b) == 0
+    with pytest.raises(RuntimeError, match="tape of this forward pass is gone"):
+        u4.sum().backward()
+    del u2
