@@ -52,7 +52,10 @@ import argparse
 import concurrent.futures as cf
 import json
 import os
+import platform
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -91,6 +94,94 @@ AD_K = 100
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "particle-steps/sec batched vs plain on 1 GPU (FP64) + % HBM/MFMA roofline"
 
+# CPUs a one-GPU job may use on the GPU host (the host's share per GPU);
+# the CPU baselines run one system per thread on that many threads
+CPU_SHARE = 16
+
+
+def cpu_threads():
+    """Threads of the CPU baselines: the CPUs this process may run on, at
+    most the one-GPU share of the host."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(CPU_SHARE, avail))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+ORACLE_SRC = os.path.join(ROOT, "oracle", "fqsb_oracle.c")
+ORACLE_BUILDS = {
+    # name -> (gcc flags, what it is)
+    "parity": (None, "oracle/_build (gcc -O3 -ffp-contract=off, generic x86-64): the parity build"),
+    "native": (
+        ["-O3", "-march=native", "-ffp-contract=fast"],
+        "gcc -O3 -march=native -ffp-contract=fast: this host's vector ISA, contractions on (timing only)",
+    ),
+}
+
+
+def _oracle_library(build):
+    """Path of an oracle build (None: the parity build of oracle/_build); the
+    native build is compiled here, for this host's CPU, into a temporary
+    directory."""
+    flags, _ = ORACLE_BUILDS[build]
+    if flags is None:
+        return None
+    out = os.path.join(tempfile.gettempdir(), f"fqsb_oracle_{build}_{os.getpid()}.so")
+    if not os.path.exists(out):
+        subprocess.run(["gcc", *flags, "-fPIC", "-std=c11", "-shared", "-o", out, ORACLE_SRC, "-lm", "-pthread"],
+                       check=True)
+    return out
+
+
+def cpu_leg(workload, args):
+    """The CPU oracle's rate on a bounded sample of `workload`, timed in a
+    child process per oracle build (the parity build and the build for this
+    host's ISA with contractions on); the faster build is the baseline."""
+    builds = {}
+    for build in ("parity", "native"):
+        env = dict(os.environ)
+        env.pop("FQO_LIB", None)
+        try:
+            lib = _oracle_library(build)
+        except (OSError, subprocess.CalledProcessError) as e:
+            builds[build] = {"error": str(e)}
+            continue
+        if lib:
+            env["FQO_LIB"] = lib
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", workload, "--cpu-seconds",
+               str(args.cpu_seconds)]
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            builds[build] = {"error": r.stderr.strip()[-400:]}
+            continue
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        res["flags"] = ORACLE_BUILDS[build][1]
+        builds[build] = res
+    ok = {k: v for k, v in builds.items() if "value" in v}
+    if not ok:
+        return {"error": builds}
+    best = max(ok, key=lambda k: ok[k]["value"])
+    out = dict(ok[best])
+    out["build"] = best
+    out["builds"] = {k: ({"value": v["value"], "flags": v["flags"]} if "value" in v else v) for k, v in builds.items()}
+    out["cpu_model"] = cpu_model()
+    out["host_cpus"] = os.cpu_count()
+    out["cpus_allowed"] = cpu_threads()
+    return out
+
+
 PARAMS = dict(
     m=1.0,
     eta=2.0 * np.sqrt(3.0) / 10.0,
@@ -126,7 +217,7 @@ def cpu_baseline(replicas=256, target_s=10.0):
     from oracle import pyoracle
 
     pyoracle.lib()
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     systems = [pyoracle.Line1d.System_Cuspy_Laplace(**dict(PARAMS, seed=r)) for r in range(replicas)]
 
     def sweep(n):
@@ -171,7 +262,7 @@ def lr_cpu_baseline(replicas=16, target_s=10.0):
     from oracle import pyoracle
 
     pyoracle.lib()
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     replicas = max(replicas, cores)
     kw = dict(LR_PARAMS)
     systems = [
@@ -240,7 +331,7 @@ def run_longrange(args):
     flop = 2.0 * LR_N * LR_N * LR_R  # per product launch
     achieved = flop / (product_ms * 1e-3) / 1e12
     plain = None if args.no_plain else lr_plain_path()
-    cpu = None if args.no_cpu else lr_cpu_baseline()
+    cpu = None if args.no_cpu else cpu_leg("longrange", args)
     return {
         "metric": METRIC,
         "value": value,
@@ -285,7 +376,7 @@ def ad_cpu_baseline(replicas=16, target_s=10.0):
     from oracle import pyoracle
 
     pyoracle.lib()
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     replicas = max(replicas, cores)
     rng = np.random.default_rng(0)
     systems = []
@@ -360,7 +451,7 @@ def run_adjoint(args):
     value = AD_R * AD_N * AD_T * args.steps / wall
     tape_bytes = 28.0 * AD_R * AD_N * AD_K  # u, v, a (FP64) and well index (int32) per particle-step
     achieved = tape_bytes / (rev_ms * 1e-3) / 1e9
-    cpu = None if args.no_cpu else ad_cpu_baseline()
+    cpu = None if args.no_cpu else cpu_leg("adjoint", args)
     return {
         "metric": METRIC,
         "value": value,
@@ -424,11 +515,12 @@ def ragged_systems(n1d=512, n2d=8, side=256, nchunk=200):
     return systems
 
 
-def ragged_nopassing(systems, max_n=8192):
-    """No-passing versions of the Cuspy-Laplace chains of the ragged workload."""
+def ragged_nopassing(systems, max_n=None):
+    """No-passing versions of the Cuspy-Laplace chains of the ragged workload
+    (the reference's no-passing minimiser exists for Cuspy + Laplace1d only)."""
     out = []
     for name, kw in systems:
-        if name == "System_Cuspy_Laplace" and kw["shape"][0] <= max_n:
+        if name == "System_Cuspy_Laplace" and (max_n is None or kw["shape"][0] <= max_n):
             out.append(("System_Cuspy_Laplace_Nopassing", dict(kw, eta=0.0, dt=0.0)))
     return out
 
@@ -439,7 +531,7 @@ def rg_cpu_baseline(target_s=10.0):
     from oracle import pyoracle
 
     pyoracle.lib()
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     systems = ragged_systems()
     rng = np.random.default_rng(1)
     pick = rng.choice(len(systems) - 8, size=cores, replace=False)
@@ -487,7 +579,7 @@ def qs_cpu_baseline(target_s=10.0):
     from oracle import pyoracle
 
     pyoracle.lib()
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     systems = [pyoracle.Line1d.System_Cuspy_Laplace(**qs_params(r)) for r in range(cores)]
     for s in systems:
         s.u_frame = 0.0
@@ -578,7 +670,7 @@ def run_quasistatic(args):
     achieved = flop / (ms * 1e-3) / 1e12
     ppt, threads = batch.launch
     plain = None if args.no_plain else qs_plain_path()
-    cpu = None if args.no_cpu else qs_cpu_baseline()
+    cpu = None if args.no_cpu else cpu_leg("quasistatic", args)
     return {
         "metric": METRIC,
         "value": work / wall,
@@ -617,6 +709,93 @@ def run_quasistatic(args):
             "flop_per_particle_step": FLOP_PER_PARTICLE_STEP_MIN,
             "kernel_ms_total": ms,
         },
+        "cpu_baseline": cpu,
+    }
+
+
+C0_EVENTS = 201  # event 0 and 100 event pairs of the examples' protocol
+
+
+def c0_cpu_baseline(target_s=10.0):
+    """The reference's CPU configuration on the oracle, whole: one system
+    (N = 1024, seed 0), event 0 and 100 event pairs of the examples'
+    protocol on one thread (the configuration is one chain: it does not
+    divide over threads)."""
+    from oracle import pyoracle
+    from tests import protocols as P
+
+    pyoracle.lib()
+    s = pyoracle.Line1d.System_Cuspy_Laplace(**qs_params(0))
+    t0 = time.perf_counter()
+    P.quasistatic(s, C0_EVENTS, eps=QS_EPS)
+    dt = time.perf_counter() - t0
+    return {
+        "value": QS_N * s.inc / dt,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"the whole configuration: one chain of {QS_N}, {C0_EVENTS} events ({s.inc} time steps), one "
+        f"thread ({dt:.2f} s)",
+        "seconds": dt,
+    }
+
+
+def run_c0(args):
+    """The reference's CPU configuration (one Line1d Cuspy-Laplace chain,
+    N = 1024, seed 0, Weibull(2, 1) increments in chunks of 200, the
+    examples' protocol with eps = 1e-3: event 0 and 100 event pairs) on the
+    GPU: through the reference's API (the examples' Python loop of
+    eventDrivenStep / minimise calls, one launch sequence per call) and as
+    one call of the device-side driver."""
+    import torch
+
+    import frictionqpotspringblock_amd as fq
+    from tests import protocols as P
+
+    torch.cuda.init()
+    warm = fq.Line1d.System_Cuspy_Laplace(**qs_params(0))
+    P.quasistatic(warm, 5, eps=QS_EPS)
+    s = fq.Line1d.System_Cuspy_Laplace(**qs_params(0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, S_loop, inc_loop = P.quasistatic(s, C0_EVENTS, eps=QS_EPS)
+    torch.cuda.synchronize()
+    wall_loop = time.perf_counter() - t0
+    d = fq.Line1d.System_Cuspy_Laplace(**qs_params(0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q = d.quasistatic(QS_EPS, C0_EVENTS)
+    torch.cuda.synchronize()
+    wall_dev = time.perf_counter() - t0
+    assert np.array_equal(q["S"], S_loop) and np.array_equal(q["inc"], inc_loop)
+    steps = int(inc_loop[-1])
+    cpu = None if args.no_cpu else cpu_leg("c0", args)
+    return {
+        "metric": METRIC,
+        "value": QS_N * steps / wall_dev,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": wall_dev * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Weibull yield landscape from pcg32 seed 0)",
+        "seconds": wall_dev,
+        "config": {
+            "workload": f"the reference's CPU configuration: one Line1d Cuspy-Laplace chain of {QS_N} (seed 0), "
+            f"{C0_EVENTS} events of the examples' protocol (eps {QS_EPS}), {steps} time steps",
+            "time_steps": steps,
+            "avalanche_sizes_sum": int(np.abs(q["S"]).sum()),
+            "device_driver_seconds": wall_dev,
+            "python_loop_seconds": wall_loop,
+        },
+        "plain_value": QS_N * steps / wall_loop,
+        "plain": "the examples' Python loop through the reference's API (System.eventDrivenStep / minimise)",
+        "roofline": None,
+        "roofline_note": "one 1024-particle chain is one workgroup on one CU: latency-bound, no roofline applies",
         "cpu_baseline": cpu,
     }
 
@@ -690,7 +869,7 @@ def run_ragged(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     roof = rg_tile_roofline()
-    cpu = None if args.no_cpu else rg_cpu_baseline()
+    cpu = None if args.no_cpu else cpu_leg("ragged", args)
     return {
         "metric": METRIC,
         "value": work / wall,
@@ -742,7 +921,11 @@ def run_flow(args):
     ev_ms = batch.flowSteps(args.steps * BLOCK, V_FRAME, block=BLOCK, timed=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    assert np.all(np.isfinite(batch.u_frame))
+    # every replica took every step, and its state is finite
+    inc = batch.inc
+    assert np.all(inc == inc[0]) and inc[0] >= BLOCK * args.steps, inc[:4]
+    u = batch.u
+    assert np.all(np.isfinite(u[:: max(1, R // 16)])) and np.all(np.isfinite(batch.u_frame))
 
     particle_steps = R * N * BLOCK * args.steps
     value = particle_steps / wall
@@ -751,7 +934,7 @@ def run_flow(args):
     achieved = flop / (kernel_ms * 1e-3) / 1e12
 
     plain = None if args.no_plain else plain_path()
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_leg("flow", args)
     ppt, threads = batch.launch
     return {
         "metric": METRIC,
@@ -836,28 +1019,81 @@ def measure_peaks():
     }
 
 
+CPU_LEGS = {
+    "flow": lambda t: cpu_baseline(target_s=t),
+    "longrange": lambda t: lr_cpu_baseline(target_s=t),
+    "adjoint": lambda t: ad_cpu_baseline(target_s=t),
+    "ragged": lambda t: rg_cpu_baseline(target_s=t),
+    "quasistatic": lambda t: qs_cpu_baseline(target_s=t),
+    "c0": lambda t: c0_cpu_baseline(target_s=t),
+}
+
+# the other configurations, measured after the headline in the default run
+# (name, function, timed bench steps, warmup steps)
+EXTRAS = [
+    ("c0", "run_c0", 1, 0),
+    ("quasistatic", "run_quasistatic", 20, 1),
+    ("ragged", "run_ragged", 10, 1),
+    ("longrange", "run_longrange", 20, 1),
+    ("adjoint", "run_adjoint", 2, 1),
+]
+
+
+def extras(args):
+    """Every other configuration at a reduced length (the same workloads; their
+    own --workload runs are the full lengths), each with its value, roofline
+    and CPU baseline."""
+    out = {}
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "seconds", "config", "plain_value", "plain",
+            "batched_over_plain", "roofline", "roofline_note", "cpu_baseline")
+    for name, fn, steps, warmup in EXTRAS:
+        a = argparse.Namespace(**vars(args))
+        a.steps, a.warmup = steps, warmup
+        t0 = time.perf_counter()
+        try:
+            line = globals()[fn](a)
+        except Exception as e:  # one configuration failing does not hide the others
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            continue
+        rec = {k: line[k] for k in keep if k in line}
+        rec["seconds_total"] = time.perf_counter() - t0
+        out[name] = rec
+        print(f"[bench] {name}: {rec.get('value', 0):.3e} particle-steps/s ({rec['seconds_total']:.1f} s)",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument(
-        "--workload", choices=["flow", "longrange", "adjoint", "ragged", "quasistatic"], default="flow"
+        "--workload", choices=["flow", "longrange", "adjoint", "ragged", "quasistatic", "c0"], default="flow"
     )
     ap.add_argument(
         "--steps", type=int, default=None,
         help="timed bench steps (flow: resident blocks of 1000 flowSteps, default 100; "
         "longrange: blocks of 100 timeSteps, default 200; adjoint: forward + backward passes, default 5; "
-        "ragged: quasistatic events, default 50; quasistatic: event pairs, default 100)",
+        "ragged: quasistatic events, default 50; quasistatic: event pairs, default 100; c0: ignored)",
     )
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU work per oracle build of a CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-peaks", action="store_true", help="skip the copy / matrix-product peak measurement")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="flow: skip the other configurations measured after the headline")
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument(
         "--copy-probe", action="store_true",
         help="only run a plain device copy of 1 GiB three times (calibration of HBM byte counters) and exit",
     )
     args = ap.parse_args()
+    if args.cpu_leg:
+        # child process of cpu_leg: one CPU baseline with the oracle build
+        # named by FQO_LIB (or the parity build)
+        print(json.dumps(CPU_LEGS[args.cpu_leg](args.cpu_seconds)), flush=True)
+        return
     if args.copy_probe:
         import torch
 
@@ -876,15 +1112,19 @@ def main():
         )
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 50, "quasistatic": 100}[args.workload]
+        args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 50, "quasistatic": 100,
+                      "c0": 1}[args.workload]
     run = {
         "flow": run_flow,
         "longrange": run_longrange,
         "adjoint": run_adjoint,
         "ragged": run_ragged,
         "quasistatic": run_quasistatic,
+        "c0": run_c0,
     }[args.workload]
     line = run(args)
+    if args.workload == "flow" and not args.no_extras:
+        line["workloads"] = extras(args)
     if not args.no_peaks:
         line["peaks_measured"] = measure_peaks()
     print(json.dumps(line), flush=True)

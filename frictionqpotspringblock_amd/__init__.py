@@ -14,6 +14,16 @@ import sys
 
 _here = os.path.dirname(os.path.abspath(__file__))
 
+# One HIP runtime per process. PyTorch ships its own libamdhip64 (loaded by
+# the name libamdhip64.so, with the soname libamdhip64.so.7); loaded first,
+# it also satisfies libfqsb.so's dependency on libamdhip64.so.7, so that
+# tensors and the library share one runtime and one view of the GPU. Loaded
+# after libfqsb.so, torch would bring a second HIP runtime that finds no GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the library itself does not need torch
+    pass
+
 try:
     from . import _fqsb
 except ImportError as e:  # pragma: no cover - exercised only without a build

@@ -314,9 +314,27 @@ def validate_checkpoint(ck, replicas, shape, nchunk, external):
 def save_events(path, q, system=None):
     """Per-event records of `quasistatic` in the examples' output schema
     (examples/Line1d_Cuspy_Laplace.py:63-67: "x_frame", "f_frame", "S"; plus
-    "inc"), as an .npz file; `system` picks one row of a batch's records."""
+    "inc"); `system` picks one row of a batch's records. A path ending in
+    .h5 / .hdf5 is written as HDF5 (the examples' format, readable by their
+    h5py scripts), any other as .npz."""
     pick = (lambda x: x) if system is None else (lambda x: np.asarray(x)[system])
-    np.savez(path, x_frame=pick(q["u_frame"]), f_frame=pick(q["f_frame"]), S=pick(q["S"]), inc=pick(q["inc"]))
+    rec = dict(x_frame=pick(q["u_frame"]), f_frame=pick(q["f_frame"]), S=pick(q["S"]), inc=pick(q["inc"]))
+    if str(path).endswith((".h5", ".hdf5")):
+        from . import h5events
+
+        h5events.write(path, rec)
+    else:
+        np.savez(path, **rec)
+
+
+def load_events(path):
+    """The records written by save_events ({"x_frame", "f_frame", "S", "inc"})."""
+    if str(path).endswith((".h5", ".hdf5")):
+        from . import h5events
+
+        return h5events.read(path)
+    with np.load(path) as f:
+        return {k: f[k] for k in f.files}
 
 
 def _check(rc):

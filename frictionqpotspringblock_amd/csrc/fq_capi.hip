@@ -93,6 +93,17 @@ constexpr size_t ADJ_TAPE_BYTES = size_t(16) << 30;
 constexpr int64_t RAGGED_RESIDENT_MAX = 2048;
 // steps per resident launch of timeSteps / flowSteps
 constexpr uint64_t STEP_BLOCK = 1u << 16;
+// long-range chains (without forcing) above this size step over tiles of
+// LR_TILE_PARTICLES particles (one per thread), each tile holding every slip
+// of the chain in LDS
+constexpr int64_t LR_RESIDENT_MAX = 2048;
+constexpr int64_t LR_TILE_PARTICLES = 256;
+// largest chain of the resident no-passing kernel (16 particles per thread on
+// 512 threads); longer chains sweep over tiles
+constexpr int64_t NOPASSING_RESIDENT_MAX = 8192;
+// sweeps between host checks of a tiled no-passing run, and particles per tile
+constexpr uint64_t NP_BLOCK = 256;
+constexpr int64_t NP_TILE_PARTICLES = 4096;
 // steps between host checks of a batched long-range run (matrix form)
 constexpr uint64_t LR_BLOCK = 128;
 // steps between host checks of a tiled run
@@ -170,6 +181,11 @@ struct Group {
     };
     std::vector<TileGraph> tile_graphs;
     bool use_graphs = true;
+    // no-passing chains beyond the resident kernel (or FQSB_NOPASSING_TILES=1):
+    // Jacobi sweeps over the tiles of `tiles` (fq_tile.hpp k_np_*), the second
+    // iterate buffer in u2 and the current wells' yield positions in yl / yr
+    bool np_tiled = false;
+    std::vector<TileGraph> np_graphs;
     // FQSB_GENERAL_KERNEL=1: timeSteps / flowSteps on k_resident instead of
     // k_flow (the two give the same results; a switch for comparisons)
     bool general_kernel = false;
@@ -230,6 +246,9 @@ struct Group {
         dfree(lr_part);
         free_adjoint();
         for (auto& tg : tile_graphs) {
+            (void)hipGraphExecDestroy(tg.exec);
+        }
+        for (auto& tg : np_graphs) {
             (void)hipGraphExecDestroy(tg.exec);
         }
         dfree(tiles);
@@ -1017,6 +1036,18 @@ struct Group {
         return ms;
     }
 
+    // minimise (detail.h:1754-1785) of every system: batches of long-range
+    // chains in matrix form run it on the FP64 matrix cores (the forces of all
+    // replicas as one product per step), others on the resident / tiled /
+    // streaming kernels
+    float minimise_run(fq::RunArgs ra, uint64_t max_iter)
+    {
+        if (lr_method == FQSB_LR_MATRIX && !ra.time_activity) {
+            return lr_run(ra, max_iter, LR_BLOCK, false);
+        }
+        return run(ra, max_iter, MIN_BLOCK, false);
+    }
+
     float steps(uint64_t n, bool flow, double v_frame, uint64_t block, bool check_each, bool timed)
     {
         fq::RunArgs ra{};
@@ -1429,7 +1460,7 @@ struct Group {
                     ra.tol = tol;
                     ra.tol2 = tol * tol;
                     ra.niter_tol = (int64_t)niter_tol;
-                    run(ra, max_iter, MIN_BLOCK, false);
+                    minimise_run(ra, max_iter);
                     for (int64_t r = 0; r < R; ++r) {
                         const auto st = hrs[(size_t)r].status;
                         if (on(r) && st != fq::ST_CONVERGED) {
@@ -1621,6 +1652,12 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
             if (limit > 0 && g->N > limit) {
                 throw Error("streaming above the resident size limit");
             }
+            // long range without forcing above LR_RESIDENT_MAX: the O(N^2) sum of
+            // a step spread over tiles (one particle per thread) instead of one
+            // workgroup
+            if (inter == FQSB_INT_LONGRANGE1D && g->ext == FQSB_EXT_NONE && g->N > LR_RESIDENT_MAX) {
+                throw Error("long range over tiles");
+            }
             g->choose_launch(0, 0);
         }
         catch (const Error&) {
@@ -1650,6 +1687,7 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
             const bool lat = inter == FQSB_INT_LAPLACE2D || inter == FQSB_INT_QUARTICGRADIENT2D;
             const int64_t cols = d.rank == 1 ? 1 : (int64_t)d.cols;
             const int64_t rows = (int64_t)d.rows;
+            const bool lr = inter == FQSB_INT_LONGRANGE1D;
             int64_t step;
             if (lat) {
                 int64_t rpt = std::max<int64_t>(1, 4096 / cols);
@@ -1659,7 +1697,7 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
             }
             else {
                 ok = N >= 3;
-                step = 4096;
+                step = lr ? LR_TILE_PARTICLES : 4096;
             }
             first[(size_t)r] = (int32_t)tl.size();
             for (int64_t p0 = 0; ok && p0 < N; p0 += step) {
@@ -1668,7 +1706,9 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
                 td.p0 = p0;
                 td.p1 = std::min<int64_t>(N, p0 + step);
                 tl.push_back(td);
-                lds = std::max<size_t>(lds, sizeof(double) * (size_t)((td.p1 - td.p0) + 2 * (lat ? cols : 1)));
+                // LDS: the tile's slips and halo (long range: every slip of the system)
+                const int64_t nl = lr ? N : (td.p1 - td.p0) + 2 * (lat ? cols : 1);
+                lds = std::max<size_t>(lds, sizeof(double) * (size_t)nl);
             }
         }
         first[(size_t)R] = (int32_t)tl.size();
@@ -1708,6 +1748,37 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
                 g->tile_fail = dalloc<int32_t>(tb.size());
             }
             g->tile_flag = dalloc<int32_t>((size_t)R);
+        }
+    }
+
+    // no-passing chains beyond the resident no-passing kernel (8192
+    // particles): sweeps over tiles of 4096 particles
+    if (g->minim == FQSB_MIN_OVERDAMPED) {
+        const char* force = std::getenv("FQSB_NOPASSING_TILES");
+        if ((force && force[0] == '1') || g->N > NOPASSING_RESIDENT_MAX) {
+            std::vector<fq::TileDesc> tl;
+            std::vector<int32_t> first((size_t)R + 1, 0);
+            for (int64_t r = 0; r < R; ++r) {
+                first[(size_t)r] = (int32_t)tl.size();
+                for (int64_t p0 = 0; p0 < Ns[(size_t)r]; p0 += NP_TILE_PARTICLES) {
+                    fq::TileDesc td{};
+                    td.sys = (int32_t)r;
+                    td.p0 = p0;
+                    td.p1 = std::min<int64_t>(Ns[(size_t)r], p0 + NP_TILE_PARTICLES);
+                    tl.push_back(td);
+                }
+            }
+            first[(size_t)R] = (int32_t)tl.size();
+            g->np_tiled = true;
+            g->ntiles = (int)tl.size();
+            g->tiles = dalloc<fq::TileDesc>(tl.size());
+            g->h2d(g->tiles, tl.data(), tl.size());
+            g->tile_first = dalloc<int32_t>((size_t)R + 1);
+            g->h2d(g->tile_first, first.data(), (size_t)R + 1);
+            g->tile_part = dalloc<double>(2 * tl.size());
+            g->u2 = dalloc<double>(RN);
+            g->yl = dalloc<double>(RN);
+            g->yr = dalloc<double>(RN);
         }
     }
 
@@ -1776,6 +1847,11 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
         S.stddev = d.stddev;
         S.seed = d.seed;
         S.offset = d.offset;
+        if (g->np_tiled) {
+            S.u_b = g->u2 + o;
+            S.yl_a = g->yl + o;
+            S.yr_a = g->yr + o;
+        }
         if (g->streaming) {
             S.u_b = g->u2 + o;
             S.v_b = g->v2 + o;
@@ -1846,6 +1922,84 @@ int64_t checked_p(const Group& g, uint64_t p)
 // No-passing minimise (detail.h:1694-1753) on the device (Jacobi sweeps) for
 // every system of the group (with g.mask: the masked-in ones). Returns per
 // system true if the StopList converged.
+// Tiled no-passing run (chains beyond the resident kernel): per sweep one
+// k_np_sweep, k_np_resid and k_np_decide for every system of the group,
+// captured in blocks of NP_BLOCK sweeps as a HIP graph (a replay continues
+// from the run state in device memory; sweeps of a stopped system return at
+// once), checked by the host after every block.
+static void nopassing_tiled(Group& g, const fq::RunArgs& ra, uint64_t max_iter)
+{
+    const int64_t target = (int64_t)std::min<uint64_t>(max_iter, (uint64_t)std::numeric_limits<int64_t>::max());
+    for (auto& r : g.hrs) {
+        r.target = target;
+        r.cur = 0;
+    }
+    g.push();
+    HIPC(fq::launch_tile_prologue(g.views, g.tiles, g.ntiles, g.stream));
+    auto enqueue = [&](uint64_t nb) {
+        for (uint64_t s = 0; s < nb; ++s) {
+            HIPC(fq::launch_np_sweep(g.views, g.tiles, g.ntiles, (int)g.R, g.tile_first, g.tile_part, ra, g.stream));
+        }
+    };
+    auto launch_block = [&](uint64_t nb) {
+        if (!g.use_graphs) {
+            enqueue(nb);
+            return;
+        }
+        for (const auto& tg : g.np_graphs) {
+            if (tg.steps == nb && std::memcmp(&tg.ra, &ra, sizeof(ra)) == 0) {
+                HIPC(hipGraphLaunch(tg.exec, g.stream));
+                return;
+            }
+        }
+        hipGraph_t graph = nullptr;
+        HIPC(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(nb);
+        }
+        catch (...) {
+            (void)hipStreamEndCapture(g.stream, &graph);
+            if (graph) {
+                (void)hipGraphDestroy(graph);
+            }
+            throw;
+        }
+        HIPC(hipStreamEndCapture(g.stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPC(e);
+        if (g.np_graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(g.np_graphs.front().exec);
+            g.np_graphs.erase(g.np_graphs.begin());
+        }
+        g.np_graphs.push_back(Group::TileGraph{ra, nb, exec});
+        HIPC(hipGraphLaunch(exec, g.stream));
+    };
+    while (true) {
+        int64_t lag = 0;
+        for (const auto& r : g.hrs) {
+            if ((r.status == fq::ST_RUNNING || r.status == fq::ST_BUDGET) && r.steps < r.target) {
+                lag = std::max<int64_t>(lag, r.target - r.steps);
+            }
+        }
+        if (lag == 0) {
+            break;
+        }
+        launch_block(std::min<uint64_t>(NP_BLOCK, (uint64_t)lag));
+        g.pull();
+    }
+    HIPC(fq::launch_np_finish(g.views, g.tiles, g.ntiles, g.stream));
+    for (auto& r : g.hrs) {
+        r.cur = 0;
+        if (r.status == fq::ST_RUNNING) {
+            r.status = fq::ST_BUDGET;
+        }
+    }
+    g.push();
+    HIPC(hipStreamSynchronize(g.stream));
+}
+
 std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter)
 {
     for (auto& r : g.hrs) {
@@ -1854,6 +2008,19 @@ std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64
     }
     g.reset_run(fq::MODE_MINIMISE);
     g.push();
+    if (g.np_tiled) {
+        fq::RunArgs ra{};
+        ra.mode = fq::MODE_MINIMISE;
+        ra.tol = tol;
+        ra.tol2 = tol * tol;
+        ra.niter_tol = (int64_t)niter_tol;
+        nopassing_tiled(g, ra, max_iter);
+        std::vector<bool> conv((size_t)g.R);
+        for (int64_t r = 0; r < g.R; ++r) {
+            conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
+        }
+        return conv;
+    }
     // particles per thread: the fewest for which the launch fits the kernel's
     // thread bound (1024 up to 8 particles per thread, 512 at 16)
     int ppt = 1;
@@ -2548,7 +2715,7 @@ int fqsb_batch_minimise(fqsb_batch* b, double tol, uint64_t niter_tol, uint64_t 
         ra.tol = tol;
         ra.tol2 = tol * tol;
         ra.niter_tol = (int64_t)niter_tol;
-        g.run(ra, max_iter, MIN_BLOCK, false);
+        g.minimise_run(ra, max_iter);
         for (int64_t r = 0; r < g.R; ++r) {
             const auto& st = g.hrs[(size_t)r];
             if (st.status == fq::ST_NAN) {

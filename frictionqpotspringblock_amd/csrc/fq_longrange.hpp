@@ -257,7 +257,11 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
 
     const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, inv_m = S.inv_m, kf = S.k_frame;
     const double uf = ra.flow ? uf0 + ra.v_frame * dt : uf0;
-    const bool last = steps + 1 >= target;
+    // minimise (detail.h:1754-1785): every step's forces are stored, since
+    // any step can be the last one (the StopList decides after the step)
+    const bool mini = ra.mode == MODE_MINIMISE;
+    const bool last = mini || steps + 1 >= target;
+    double sf = 0.0, sff = 0.0;
 
     if constexpr (EXT) {
         uint64_t rng = rng0;
@@ -339,6 +343,51 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
             S.f_damp[p] = meta * vv;
             S.f[p] = f;
         }
+        sf += f * f;
+        sff += ff * ff;
+    }
+    int32_t stop = 0;
+    if (mini) {
+        // residual (detail.h:1512-1520) with the norms in a fixed order, and
+        // the StopList (GooseFEM Iterate::StopList, as in k_resident)
+        __shared__ double s_red[2][RES_MAX_WAVES];
+        const int lane = t & 63, wave = t >> 6, nw = T >> 6;
+        sf = wave_sum(sf);
+        sff = wave_sum(sff);
+        if (lane == 0) {
+            s_red[0][wave] = sf;
+            s_red[1][wave] = sff;
+        }
+        __syncthreads();
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            a0 += s_red[0][w];
+            a1 += s_red[1][w];
+        }
+        const double rf = __builtin_sqrt(a0), rff = __builtin_sqrt(a1);
+        const double r = (rff != 0.0) ? rf / rff : rf;
+        const int32_t below = !(r >= ra.tol) ? uniform_i(rs->below) + 1 : 0;
+        const int32_t below2 = !(r >= ra.tol2) ? uniform_i(rs->below2) + 1 : 0;
+        const int32_t dec = !(r >= uniform_d(rs->prev)) ? uniform_i(rs->dec) + 1 : 0;
+        const int64_t nt = ra.niter_tol;
+        stop = (nt <= 0 || (dec >= nt - 1 && below >= nt) || below2 >= nt) ? 1 : 0;
+        __syncthreads(); // every thread read the counters before thread 0 updates them
+        if (t == 0) {
+            rs->below = below;
+            rs->below2 = below2;
+            rs->dec = dec;
+            rs->prev = r;
+        }
+        if (stop && !uniform_i(s_flag[1])) {
+            // quench (detail.h:1527-1532): v = a = 0 and the forces again
+            for (int64_t p = t; p < N; p += T) {
+                S.v[p] = 0.0;
+                S.a[p] = 0.0;
+                const double fd = meta * 0.0;
+                S.f_damp[p] = fd;
+                S.f[p] = S.f_frame[p] + S.f_pot[p] + S.f_int[p] + fd;
+            }
+        }
     }
     if (t == 0) {
         rs->inc = inc;
@@ -346,6 +395,9 @@ __global__ void __launch_bounds__(1024) k_lr_elem(const SysView* __restrict__ vi
         rs->steps = steps + 1;
         if (uniform_i(s_flag[1])) {
             rs->status = ST_NAN;
+        }
+        else if (stop) {
+            rs->status = ST_CONVERGED;
         }
     }
 }

@@ -8,12 +8,14 @@ namespace fq {
     X(P, INT_NONE) X(P, INT_LAPLACE1D) X(P, INT_QUARTIC1D) X(P, INT_QUARTICGRADIENT1D) X(P, INT_LAPLACE2D)    \
         X(P, INT_QUARTICGRADIENT2D)
 
+// long range: the reference's class has the Cuspy potential only
+#define FQ_TILE_LR(X) X(POT_CUSPY, INT_LONGRANGE1D)
+
 bool tile_supported(int pot, int inter, int ext)
 {
-    (void)pot;
     return ext == 0 && (inter == INT_NONE || inter == INT_LAPLACE1D || inter == INT_QUARTIC1D ||
                         inter == INT_QUARTICGRADIENT1D || inter == INT_LAPLACE2D ||
-                        inter == INT_QUARTICGRADIENT2D);
+                        inter == INT_QUARTICGRADIENT2D || (inter == INT_LONGRANGE1D && pot == POT_CUSPY));
 }
 
 hipError_t launch_tile_prologue(const SysView* views, const TileDesc* tiles, int ntiles, hipStream_t stream)
@@ -34,6 +36,7 @@ bool launch_tile_step(int pot, int inter, const SysView* views, const TileDesc* 
     FQ_TILE_INTS(FQ_STEP, POT_CUSPY)
     FQ_TILE_INTS(FQ_STEP, POT_SEMISMOOTH)
     FQ_TILE_INTS(FQ_STEP, POT_SMOOTH)
+    FQ_TILE_LR(FQ_STEP)
 #undef FQ_STEP
     return false;
 }
@@ -92,11 +95,30 @@ bool launch_tile_finish(int pot, int inter, const SysView* views, const TileDesc
     FQ_TILE_INTS(FQ_FIN, POT_CUSPY)
     FQ_TILE_INTS(FQ_FIN, POT_SEMISMOOTH)
     FQ_TILE_INTS(FQ_FIN, POT_SMOOTH)
+    FQ_TILE_LR(FQ_FIN)
 #undef FQ_FIN
     if (ok) {
         hipLaunchKernelGGL(k_tile_finish_u, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
     }
     return ok;
+}
+
+// no-passing sweeps of chains over tiles (Cuspy + Laplace1d)
+hipError_t launch_np_sweep(const SysView* views, const TileDesc* tiles, int ntiles, int nsys,
+                           const int32_t* tile_first, double* part, const RunArgs& ra, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_np_sweep, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
+    hipLaunchKernelGGL(k_np_resid, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles, part);
+    hipLaunchKernelGGL(k_np_decide, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, stream, views, nsys,
+                       tile_first, part, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_np_finish(const SysView* views, const TileDesc* tiles, int ntiles, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_np_finish, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
+    hipLaunchKernelGGL(k_tile_finish_u, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, stream, views, tiles);
+    return hipGetLastError();
 }
 
 } // namespace fq

@@ -107,9 +107,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_tile_step(const SysView* __res
     const double uf = ra.flow ? uniform_d(rs->u_frame) + ra.v_frame * dt : uniform_d(rs->u_frame);
 
     // slips of the tile and its halo into LDS: local index k of global q
-    const int64_t h = LAT ? cols : 1;  // halo width
+    // (long range: every slip of the system, at its own index)
+    constexpr bool LR = INT == INT_LONGRANGE1D;
+    const int64_t h = LR ? p0 : (LAT ? cols : 1); // halo width (long range: the slips before the tile)
     const int64_t n = p1 - p0;
-    const int64_t len = n + 2 * h;
+    const int64_t len = LR ? N : n + 2 * h;
     for (int64_t k = t; k < len; k += T) {
         int64_t q = p0 - h + k;
         q = q < 0 ? q + N : (q >= N ? q - N : q);
@@ -171,6 +173,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_tile_step(const SysView* __res
         double fi;
         if constexpr (INT == INT_NONE) {
             fi = 0.0;
+        }
+        else if constexpr (LR) {
+            // the reference's sequential sum over all particles (detail.h:
+            // 849-867), slips from LDS (one address per step of the sum:
+            // a broadcast), prefactors from the system's table
+            const double* __restrict__ pref = S.pref;
+            fi = longrange_force(S, [&](int64_t i) { return s_x[i]; }, [&](int64_t d) { return pref[d]; }, p);
         }
         else if constexpr (LAT) {
             fi = lattice_force<INT>(S, X, p / cols, p % cols);
@@ -683,6 +692,10 @@ __global__ void __launch_bounds__(TILE_THREADS) k_tile_finish(const SysView* __r
             if constexpr (INT == INT_NONE) {
                 fi = 0.0;
             }
+            else if constexpr (INT == INT_LONGRANGE1D) {
+                const double* __restrict__ pref = S.pref;
+                fi = longrange_force(S, U, [&](int64_t d) { return pref[d]; }, p);
+            }
             else if constexpr (LAT) {
                 fi = lattice_force<INT>(S, U, p / cols, p % cols);
             }
@@ -715,6 +728,198 @@ __global__ void __launch_bounds__(TILE_THREADS) k_tile_finish_u(const SysView* _
     }
     for (int64_t p = td.p0 + threadIdx.x; p < td.p1; p += blockDim.x) {
         S.u[p] = S.u_b[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// No-passing minimiser (Jacobi sweep, Cuspy + Laplace1d, detail.h:1694-1753)
+// for chains of any length, over tiles of TILE_PARTICLES particles, one
+// launch per kernel per sweep:
+//   k_np_sweep   : every particle of the tile jumps to the minimum of its
+//                  quadratic well given the previous iterate of its
+//                  neighbours (read from the state buffer, the reference's
+//                  snapshot `m_v_n`), walking its well until it is stable,
+//                  with the landscape window moved in place by its owner
+//                  (align_particle: the same steps as the resident kernel);
+//                  the new iterate goes into the other buffer;
+//   k_np_resid   : per tile, the partial sums of |f|^2 and |f_frame|^2 of the
+//                  new iterate (f = f_potential + f_interactions + f_frame,
+//                  detail.h:1741);
+//   k_np_decide  : per system, the StopList on the residual (tile sums in a
+//                  fixed order), then the buffer flip.
+// The well of each particle only depends on its own slip, so the in-place
+// walk of a sweep never has to be undone: a sweep that the StopList stops
+// after is the last one, and sweeps after a stop are not launched (the
+// kernels return on a stopped system). k_np_finish writes the final iterate
+// and its forces (quenched on convergence, as the resident kernel does).
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(TILE_THREADS) k_np_sweep(const SysView* __restrict__ views,
+                                                            const TileDesc* __restrict__ tiles)
+{
+    const TileDesc td = tiles[blockIdx.x];
+    const SysView& S = views[td.sys];
+    const RunState* const rs = S.rs;
+    const int32_t status = uniform_i(rs->status);
+    const int64_t steps = uniform_ll(rs->steps), target = uniform_ll(rs->target);
+    if ((status != ST_RUNNING && status != ST_BUDGET) || steps >= target) {
+        return;
+    }
+    const int cb = uniform_i(rs->cur);
+    const double* const Uc = cb ? S.u_b : S.u;
+    double* const Un = cb ? S.u : S.u_b;
+    const Landscape& L = S.ls;
+    const int64_t N = S.N;
+    const double k = S.k, kf = S.k_frame, mu = S.mu;
+    const double uf = uniform_d(rs->u_frame);
+    for (int64_t p = td.p0 + threadIdx.x; p < td.p1; p += blockDim.x) {
+        const double uneigh = Uc[p == 0 ? N - 1 : p - 1] + Uc[p == N - 1 ? 0 : p + 1];
+        double yl = S.yl_a[p], yr = S.yr_a[p];
+        int32_t j = L.jloc[p];
+        bool moved = false; // the well or the window changed
+        double un;
+        // the well sequence is monotone (the minimum moves with the well), so
+        // this terminates; the cap only bounds the work
+        for (int guard = 0; guard < (1 << 20); ++guard) {
+            const double umin = 0.5 * (yl + yr);
+            un = (k * uneigh + kf * uf + mu * umin) / (2 * k + kf + mu);
+            const int32_t jold = j;
+            moved |= align_particle(L, p, un, yl, yr, j) != 0;
+            if (j == jold) {
+                break;
+            }
+            moved = true;
+        }
+        Un[p] = un;
+        if (moved) {
+            L.jloc[p] = j;
+            S.yl_a[p] = yl;
+            S.yr_a[p] = yr;
+        }
+    }
+}
+
+// Forces of the iterate x at particle p (f_potential, f_frame, f_interactions
+// as the reference computes them after a sweep, detail.h:1735-1741).
+__device__ __forceinline__ void np_forces(const SysView& S, const double* __restrict__ x, int64_t p, double uf,
+                                          double& fp, double& ff, double& fi)
+{
+    const int64_t N = S.N;
+    const double u = x[p];
+    const double umin = 0.5 * (S.yl_a[p] + S.yr_a[p]);
+    ff = S.k_frame * (uf - u);
+    fp = S.mu * (umin - u);
+    fi = chain_force<INT_LAPLACE1D>(S, x[p == 0 ? N - 1 : p - 1], u, x[p == N - 1 ? 0 : p + 1]);
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_np_resid(const SysView* __restrict__ views,
+                                                            const TileDesc* __restrict__ tiles,
+                                                            double* __restrict__ part)
+{
+    __shared__ double s_red[2][TILE_THREADS / 64];
+    const TileDesc td = tiles[blockIdx.x];
+    const SysView& S = views[td.sys];
+    const RunState* const rs = S.rs;
+    const int32_t status = uniform_i(rs->status);
+    const int64_t steps = uniform_ll(rs->steps), target = uniform_ll(rs->target);
+    if ((status != ST_RUNNING && status != ST_BUDGET) || steps >= target) {
+        return;
+    }
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+    const double* const Un = uniform_i(rs->cur) ? S.u : S.u_b; // the new iterate
+    const double uf = uniform_d(rs->u_frame);
+    double sf = 0.0, sff = 0.0;
+    for (int64_t p = td.p0 + t; p < td.p1; p += blockDim.x) {
+        double fp, ff, fi;
+        np_forces(S, Un, p, uf, fp, ff, fi);
+        const double f = fp + fi + ff;
+        sf += f * f;
+        sff += ff * ff;
+    }
+    sf = wave_sum(sf);
+    sff = wave_sum(sff);
+    if (lane == 0) {
+        s_red[0][wave] = sf;
+        s_red[1][wave] = sff;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            a0 += s_red[0][w];
+            a1 += s_red[1][w];
+        }
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// One thread per system: commit the sweep (flip the buffer) and push its
+// residual through the StopList (detail.h:1743-1750).
+__global__ void __launch_bounds__(64) k_np_decide(const SysView* __restrict__ views, int nsys,
+                                                  const int32_t* __restrict__ tile_first,
+                                                  const double* __restrict__ part, RunArgs ra)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) {
+        return;
+    }
+    const SysView& S = views[s];
+    RunState* const rs = S.rs;
+    if ((rs->status != ST_RUNNING && rs->status != ST_BUDGET) || rs->steps >= rs->target) {
+        return;
+    }
+    rs->cur ^= 1;
+    rs->steps += 1;
+    double sf = 0.0, sff = 0.0;
+    for (int k = tile_first[s]; k < tile_first[s + 1]; ++k) {
+        sf += part[2 * k];
+        sff += part[2 * k + 1];
+    }
+    const double rf = __builtin_sqrt(sf), rff = __builtin_sqrt(sff);
+    const double r = (rff != 0.0) ? rf / rff : rf;
+    rs->below = !(r >= ra.tol) ? rs->below + 1 : 0;
+    rs->below2 = !(r >= ra.tol2) ? rs->below2 + 1 : 0;
+    rs->dec = !(r >= rs->prev) ? rs->dec + 1 : 0;
+    rs->prev = r;
+    const int64_t nt = ra.niter_tol;
+    if (nt <= 0 || (rs->dec >= nt - 1 && rs->below >= nt) || rs->below2 >= nt) {
+        rs->status = ST_CONVERGED;
+    }
+}
+
+// The final iterate and its forces into the primary arrays (after at least
+// one sweep; quenched on convergence: v = a = 0, f_damping = -eta * 0, f in
+// the order of computeForce, detail.h:1321-1330).
+__global__ void __launch_bounds__(TILE_THREADS) k_np_finish(const SysView* __restrict__ views,
+                                                             const TileDesc* __restrict__ tiles)
+{
+    const TileDesc td = tiles[blockIdx.x];
+    const SysView& S = views[td.sys];
+    const RunState* rs = S.rs;
+    if (uniform_ll(rs->steps) == 0) {
+        return;
+    }
+    const double* const Uc = uniform_i(rs->cur) ? S.u_b : S.u;
+    const bool conv = uniform_i(rs->status) == ST_CONVERGED;
+    const double uf = uniform_d(rs->u_frame);
+    const double meta = -S.eta;
+    for (int64_t p = td.p0 + threadIdx.x; p < td.p1; p += blockDim.x) {
+        double fp, ff, fi;
+        np_forces(S, Uc, p, uf, fp, ff, fi);
+        S.f_pot[p] = fp;
+        S.f_frame[p] = ff;
+        S.f_int[p] = fi;
+        if (conv) {
+            S.v[p] = 0.0;
+            S.a[p] = 0.0;
+            const double fd = meta * 0.0;
+            S.f_damp[p] = fd;
+            S.f[p] = ff + fp + fi + fd;
+        }
+        else {
+            S.f[p] = fp + fi + ff;
+        }
     }
 }
 

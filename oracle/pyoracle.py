@@ -19,7 +19,9 @@ import types
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libfqsb_oracle.so")
+# FQO_LIB: another build of fqsb_oracle.c (bench.py times a build for the
+# host's own instruction set beside the portable parity build)
+_LIB_PATH = os.environ.get("FQO_LIB") or os.path.join(_HERE, "_build", "libfqsb_oracle.so")
 
 POT = {"Cuspy": 0, "SemiSmooth": 1, "Smooth": 2}
 INT = {

@@ -25,6 +25,24 @@ def wells(y, x):
     return j
 
 
+def walk(y, x, j):
+    """The same wells as `wells`, found by walking from the indices j of the
+    previous step (a step moves a particle by a few wells at most)."""
+    R, N, n = y.shape
+    rows = np.arange(R)[:, None]
+    cols = np.arange(N)[None, :]
+    j = j.copy()
+    while True:
+        up = (j < n - 2) & (x >= y[rows, cols, np.minimum(j + 1, n - 1)])
+        down = (j > 0) & (x < y[rows, cols, j])
+        if not (np.any(up) or np.any(down)):
+            break
+        j += up
+        j -= down
+    assert np.all(y[rows, cols, j] <= x) and np.all(x < y[rows, cols, j + 1])
+    return j
+
+
 def lap(x):
     """(x[p-1] - 2 x[p] + x[p+1]), periodic, in the reference's order."""
     return np.roll(x, 1, axis=1) - 2 * x + np.roll(x, -1, axis=1)
@@ -48,11 +66,12 @@ def run(u, v, a, y, par, uf, n, dt, m, flow=False, v_frame=0.0, tangent=None):
     if tangent is not None:
         du, dv, da, dpar, dy = (np.array(x, dtype=float) for x in tangent)
         dmu, dk, dkf, deta = (dpar[:, q : q + 1] for q in range(4))
+    j = None
     for _ in range(n):
         if flow:
             uf = uf + v_frame * dt
         u1 = u + dt * v + c2 * a
-        j = wells(y, u1)
+        j = wells(y, u1) if j is None else walk(y, u1, j)
         yl = y[rows, cols, j]
         yr = y[rows, cols, j + 1]
         lu = lap(u1)

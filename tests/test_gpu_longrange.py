@@ -174,3 +174,44 @@ def test_errors():
     mat.u = np.full((2, 100), np.nan)
     with pytest.raises(RuntimeError, match="NaN entries found"):
         mat.timeSteps(3)
+
+
+def test_matrix_minimise_and_events():
+    """minimise (detail.h:1754-1785) and the examples' quasistatic protocol
+    of a batch in matrix form against the exact sequential sum: the same stop
+    steps, avalanche sizes and wells, u within 1e-9 (the forces of the two
+    forms differ by a few ulp per step, see the module docstring)."""
+    mat, ext = _pair(4, 1000, nchunk=100)
+    rng = np.random.default_rng(3)
+    v0 = rng.normal(size=(4, 1000))
+    for b in (mat, ext):
+        b.v = v0
+    rm, re = mat.minimise(), ext.minimise()
+    assert np.all(rm == 0) and np.all(re == 0)
+    assert np.array_equal(mat.inc, ext.inc)
+    assert np.array_equal(mat.index_at_align, ext.index_at_align)
+    assert np.allclose(mat.u, ext.u, rtol=1e-9, atol=1e-9)
+    assert np.all(mat.v == 0) and np.all(mat.a == 0)  # quenched
+    qm = mat.quasistatic(1e-3, 9, first_event=1)
+    qe = ext.quasistatic(1e-3, 9, first_event=1)
+    assert np.array_equal(qm["S"], qe["S"])
+    assert np.array_equal(qm["inc"], qe["inc"])
+    assert np.allclose(qm["u_frame"], qe["u_frame"], rtol=1e-12)
+    assert np.any(qm["S"] != 0)
+    assert np.allclose(mat.u, ext.u, rtol=1e-9, atol=1e-9)
+    assert np.allclose(mat.f, ext.f, rtol=1e-6, atol=1e-9)
+
+
+def test_matrix_quasistatic_full_size():
+    """The long-range configuration's chain (N = 8192, alpha 1.5, without the
+    forcing, whose class has no minimise): a few events of the examples'
+    protocol in matrix form equal the exact sum (tiled, one particle per
+    thread) in avalanche sizes and wells."""
+    print("long range quasistatic: R=2 N=8192", flush=True)
+    mat, ext = _pair(2, 8192)
+    qm = mat.quasistatic(1e-3, 5)
+    qe = ext.quasistatic(1e-3, 5)
+    assert np.array_equal(qm["S"], qe["S"])
+    assert np.array_equal(qm["inc"], qe["inc"])
+    assert np.array_equal(mat.index_at_align, ext.index_at_align)
+    assert np.allclose(mat.u, ext.u, rtol=1e-9, atol=1e-9)

@@ -36,8 +36,10 @@ def path(request, monkeypatch):
     monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
     monkeypatch.delenv("FQSB_NO_TILES", raising=False)
     monkeypatch.delenv("FQSB_TILE_K", raising=False)
+    monkeypatch.delenv("FQSB_NOPASSING_TILES", raising=False)
     if request.param in ("tiled", "tiled_step", "streaming"):
         monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
+        monkeypatch.setenv("FQSB_NOPASSING_TILES", "1")  # no-passing systems: sweeps over tiles
     if request.param == "tiled_step":
         monkeypatch.setenv("FQSB_TILE_K", "0")
     if request.param == "streaming":
@@ -165,18 +167,65 @@ def test_until_event_and_truncate(fq, oracle, path):
     assert np.array_equal(a.u, b.u)
 
 
-def test_nopassing(fq, oracle):
-    par = _base(800, k_interactions=1.0)
+@pytest.mark.parametrize("N,tiles", [(800, False), (800, True), (20000, False), (65536, False)])
+def test_nopassing(fq, oracle, monkeypatch, N, tiles):
+    """The no-passing minimiser (detail.h:1694-1753) on the resident kernel
+    (N <= 8192) and on sweeps over tiles of 4096 particles (N = 20000 and
+    65536, and N = 800 forced onto the tiles), bit for bit, through events
+    with refills of the 60-well windows."""
+    monkeypatch.delenv("FQSB_NOPASSING_TILES", raising=False)
+    if tiles:
+        monkeypatch.setenv("FQSB_NOPASSING_TILES", "1")
+    print(f"no-passing N={N} tiles={tiles or N > 8192}", flush=True)
+    par = _base(N, k_interactions=1.0)
     for k in ("m", "eta", "dt"):
         par.pop(k)
     a = fq.Line1d.System_Cuspy_Laplace_Nopassing(**par)
     b = oracle.Line1d.System_Cuspy_Laplace_Nopassing(**par)
-    for step in range(6):
+    for step in range(8):
         for s in (a, b):
             if step:
-                s.eventDrivenStep(1e-3, step % 2 == 0)
+                s.eventDrivenStep(1e-3 if step < 6 else 0.5, step % 2 == 0)
             assert s.minimise() == 0
         assert_same_state(a, b, f"no-passing step {step}")
+        assert np.array_equal(a.chunk.data, b.chunk.data), step
+        assert np.array_equal(a.chunk.start, b.chunk.start), step
+    # max_iter is honoured (the reference's error and return value; the
+    # StopList needs niter_tol = 10 residuals, so 3 sweeps never converge)
+    for s in (a, b):
+        s.eventDrivenStep(0.5, True)
+        with pytest.raises(RuntimeError, match="No convergence found"):
+            s.minimise(max_iter=3)
+    assert_same_state(a, b, "no-passing after max_iter")
+    assert a.minimise(max_iter=3, max_iter_is_error=False) == b.minimise(max_iter=3, max_iter_is_error=False) == 4
+    assert_same_state(a, b, "no-passing after max_iter, no error")
+
+
+@pytest.mark.parametrize("N,steps", [(2500, 100), (8192, 12)])
+def test_longrange_tiled(fq, oracle, N, steps):
+    """System_Cuspy_LongRange beyond 2048 particles steps over tiles of 256
+    particles, each thread summing its particle's long-range force over the
+    whole chain in the reference's order (detail.h:849-867): bit for bit
+    through flowSteps, minimise and a kick + minimise (N = 2500), and through
+    timeSteps at the long-range configuration's N = 8192."""
+    print(f"long range tiled N={N}", flush=True)
+    extra = dict(k_interactions=1.0, alpha=1.5)
+    a = _make(fq, "Line1d", "System_Cuspy_LongRange", N, extra)
+    b = _make(oracle, "Line1d", "System_Cuspy_LongRange", N, extra)
+    v0 = np.random.default_rng(0).normal(size=N)
+    for s in (a, b):
+        s.v = v0
+        s.flowSteps(steps, 0.05)
+    assert_same_state(a, b, "long range flowSteps")
+    if N > 3000:
+        return
+    for s in (a, b):
+        assert s.minimise() == 0
+    assert_same_state(a, b, "long range minimise")
+    for s in (a, b):
+        s.eventDrivenStep(1e-3, True)
+        assert s.minimise() == 0
+    assert_same_state(a, b, "long range kick + minimise")
 
 
 @pytest.mark.parametrize("cls", ["System_Cuspy_Laplace_RandomForcing", "System_Cuspy_Quartic_RandomForcing"])

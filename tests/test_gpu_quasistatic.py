@@ -373,3 +373,57 @@ def test_batch_checkpoint_restart(fq, tmp_path, forcing):
         save_events(tmp_path / "events.npz", qa, system=1)
         with np.load(tmp_path / "events.npz") as f:
             assert np.array_equal(f["S"], qa["S"][1]) and np.array_equal(f["x_frame"], qa["u_frame"][1])
+
+
+def test_restart_equals_uninterrupted_oracle(fq, oracle, tmp_path):
+    """A batch that runs 11 events, is written to disk, and continues in a
+    fresh process-level object from the file for 16 more events equals oracle
+    systems (seeds 3, 4, 5) that ran the 27 events without stopping: the
+    restart of the reference (chunk.restore and the state setters,
+    test_Line1d.py:309-341) reproduces the uninterrupted run, refills of the
+    40-well windows included; the event log is written as HDF5 in the
+    examples' schema and read back."""
+    from frictionqpotspringblock_amd import h5events
+    from frictionqpotspringblock_amd.batch import Batch, load_events, save_events
+
+    R, N = 3, 300
+    par = dict(
+        m=1.0,
+        eta=2.0 * np.sqrt(3.0) / 10.0,
+        mu=1.0,
+        k_interactions=1.0,
+        k_frame=1.0 / N,
+        dt=0.1,
+        shape=[N],
+        seed=3,
+        distribution="weibull",
+        parameters=[2.0, 1.0, 0.0],
+        offset=-100.0,
+        nchunk=40,
+    )
+    b = Batch(R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **par)
+    q1 = b.quasistatic(1e-3, 11)
+    b.save(tmp_path / "ck.npz")
+    del b
+    fresh = Batch(R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **par)
+    fresh.load(tmp_path / "ck.npz")
+    q2 = fresh.quasistatic(1e-3, 16, first_event=11)
+    singles = [oracle.Line1d.System_Cuspy_Laplace(**dict(par, seed=3 + r)) for r in range(R)]
+    u, idx = fresh.u, fresh.index_at_align
+    for r, s in enumerate(singles):
+        uf, ff, S, inc = P.quasistatic(s, 27)
+        assert np.array_equal(np.concatenate([q1["S"][r], q2["S"][r]]), S), r
+        assert np.array_equal(np.concatenate([q1["inc"][r], q2["inc"][r]]), inc), r
+        assert np.array_equal(np.concatenate([q1["u_frame"][r], q2["u_frame"][r]]), uf), r
+        assert np.array_equal(u[r], s.u), r
+        assert np.array_equal(idx[r], s.chunk.index_at_align), r
+    with np.load(tmp_path / "ck.npz") as f:
+        assert np.any(fresh.checkpoint()["start"] != f["start"])  # windows moved after the restart
+    try:
+        h5events.library()
+    except RuntimeError:
+        return
+    q = {k: np.concatenate([q1[k], q2[k]], axis=1) for k in q1}
+    save_events(tmp_path / "events.h5", q, system=2)
+    back = load_events(tmp_path / "events.h5")
+    assert np.array_equal(back["S"], q["S"][2]) and np.array_equal(back["x_frame"], q["u_frame"][2])

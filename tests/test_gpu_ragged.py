@@ -160,9 +160,14 @@ def test_ragged_masks_and_errors(oracle):
         rg.minimise(max_iter=3, active=~smooth)
     with pytest.raises(RuntimeError, match="unknown system class"):
         Ragged([("System_Nothing", _par(10, 0))])
-    # the no-passing kernel holds up to 8192 particles (16 per thread, 512 threads)
-    big = Ragged([("System_Cuspy_Laplace_Nopassing", dict(_par(9000, 0), eta=0.0, dt=0.0))])
-    with pytest.raises(RuntimeError, match="too large for the no-passing kernel"):
-        big.minimise()
-    ok = Ragged([("System_Cuspy_Laplace_Nopassing", dict(_par(8192, 0), eta=0.0, dt=0.0))])
-    assert ok.minimise()[0] == 0
+    # no-passing chains on both sides of the resident kernel's limit (8192
+    # particles, 16 per thread on 512 threads; longer ones sweep over tiles)
+    for n in (8192, 9000):
+        kw = dict(_par(n, 0), eta=0.0, dt=0.0)
+        rgn = Ragged([("System_Cuspy_Laplace_Nopassing", kw)])
+        o = _oracle_system(oracle, "System_Cuspy_Laplace_Nopassing", kw)
+        for s in (rgn, o):
+            s.eventDrivenStep(1e-2, True)
+        assert rgn.minimise()[0] == o.minimise() == 0
+        _assert_same(rgn, [o], f"no-passing {n}")
+        assert rgn.last_steps[0] > 10
