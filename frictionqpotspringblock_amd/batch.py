@@ -177,6 +177,11 @@ SIGNATURES = {
         [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
          _ip],
     ),
+    "fqsb_batch_quasistatic_noalloc": (
+        C.c_int,
+        [_P, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64, _ip, _dp, _dp,
+         _ip],
+    ),
     "fqsb_batch_quasistatic_stats": (C.c_int, [_P, _fp, _ip]),
     "fqsb_batch_minimise_truncate": (
         C.c_int,

@@ -29,6 +29,23 @@
 #include "../../include/fqsb.h"
 #include "fq_launch.hpp"
 
+#include <mutex>
+
+// Device memory of fqsb_batch_quasistatic_noalloc: the per-event records
+// (S, u_frame, mean f_frame, inc: 32 bytes per system and event) that
+// fqsb_batch_quasistatic stages in buffers it allocates per call. Every GPU
+// holds its own copy until the process ends; calls that overlap on one GPU
+// share it and race.
+constexpr size_t QS_SCRATCH_BYTES = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[QS_SCRATCH_BYTES];
+
+__global__ void k_zero_words(uint64_t* __restrict__ p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        p[i] = 0;
+    }
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -1258,8 +1275,12 @@ struct Group {
     // device (k_quasi, one launch per step budget, refill or end); the others
     // are driven event by event from here. Returns the time around the device
     // work (ms).
+    // `records`: device memory for the records (4 x R x nevents words, no
+    // allocation in the call; the device-side driver only), or null to
+    // allocate it here.
     float quasistatic(int64_t first, int64_t nevents, double eps, int direction, int kicks, double tol,
-                      uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* uf, double* ff, int64_t* inc_out)
+                      uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* uf, double* ff, int64_t* inc_out,
+                      void* records = nullptr)
     {
         if (first < 0 || nevents < 0) {
             throw Error("the first event and the number of events must be non-negative");
@@ -1288,10 +1309,29 @@ struct Group {
             return 0.0f;
         }
         const size_t n = (size_t)(R * nevents);
-        int64_t* dS = dalloc<int64_t>(n);
-        double* duf = dalloc<double>(n);
-        double* dff = dalloc<double>(n);
-        int64_t* dinc = dalloc<int64_t>(n);
+        const bool own = records == nullptr;
+        int64_t *dS, *dinc;
+        double *duf, *dff;
+        if (own) {
+            dS = dalloc<int64_t>(n);
+            duf = dalloc<double>(n);
+            dff = dalloc<double>(n);
+            dinc = dalloc<int64_t>(n);
+        }
+        else {
+            if (streaming || minim != FQSB_MIN_INERTIAL) {
+                throw Error("the quasistatic protocol without allocation needs the device-side driver "
+                            "(systems of one resident workgroup, inertial minimisation)");
+            }
+            dS = (int64_t*)records;
+            duf = (double*)(dS + n);
+            dff = duf + n;
+            dinc = (int64_t*)(dff + n);
+            // the records start at zero, as the allocated ones do
+            hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((4 * n + 255) / 256, 1024)), dim3(256),
+                               0, stream, (uint64_t*)records, 4 * n);
+            HIPC(hipGetLastError());
+        }
         fq::QsArgs qa{};
         qa.first = first;
         qa.nevents = nevents;
@@ -1308,8 +1348,12 @@ struct Group {
         try {
             set_iref_current();
             const bool device = !streaming && minim == FQSB_MIN_INERTIAL &&
-                                std::getenv("FQSB_HOST_QUASISTATIC") == nullptr;
+                                (!own || std::getenv("FQSB_HOST_QUASISTATIC") == nullptr);
             if (!device || !quasistatic_device(qa, tol, niter_tol, ms)) {
+                if (!own) {
+                    throw Error("the quasistatic protocol without allocation needs the device-side driver, which "
+                                "is not compiled for this system class");
+                }
                 ms = quasistatic_host(qa, tol, niter_tol, max_iter);
             }
             auto fetch = [&](auto* host, const auto* dev) {
@@ -1324,16 +1368,20 @@ struct Group {
         }
         catch (...) {
             (void)hipStreamSynchronize(stream);
+            if (own) {
+                dfree(dS);
+                dfree(duf);
+                dfree(dff);
+                dfree(dinc);
+            }
+            throw;
+        }
+        if (own) {
             dfree(dS);
             dfree(duf);
             dfree(dff);
             dfree(dinc);
-            throw;
         }
-        dfree(dS);
-        dfree(duf);
-        dfree(dff);
-        dfree(dinc);
         qs_ms = ms;
         return ms;
     }
@@ -2755,6 +2803,56 @@ int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, 
         b->g->adj_valid = false;
         b->g->quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame,
                           inc);
+    });
+}
+
+// Address of `scratch` on each device, taken once per device with that device
+// current (the array's name is a host-side stand-in, not a device address).
+static std::once_flag g_scratch_once[64];
+static void* g_scratch_ptr[64];
+static hipError_t g_scratch_err[64];
+
+static void* scratch_on(int device)
+{
+    if (device < 0 || device >= 64) {
+        throw Error("device index out of range for the quasistatic scratch");
+    }
+    std::call_once(g_scratch_once[device], [device] {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) {
+            e = hipSetDevice(device);
+        }
+        void* p = nullptr;
+        if (e == hipSuccess) {
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        }
+        if (cur != device) {
+            (void)hipSetDevice(cur);
+        }
+        g_scratch_err[device] = e;
+        g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    });
+    HIPC(g_scratch_err[device]);
+    return g_scratch_ptr[device];
+}
+
+int fqsb_batch_quasistatic_noalloc(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+                                   int kicks, double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S,
+                                   double* u_frame, double* f_frame, int64_t* inc)
+{
+    return guard([&] {
+        Group& g = *b->g;
+        const size_t need = 4 * sizeof(int64_t) * (size_t)g.R * (size_t)std::max<int64_t>(nevents, 0);
+        if (need > QS_SCRATCH_BYTES) {
+            throw Error("fqsb_batch_quasistatic_noalloc holds at most " +
+                        std::to_string(QS_SCRATCH_BYTES / (4 * sizeof(int64_t))) +
+                        " records (replicas x events) per call; " + std::to_string(g.R * nevents) + " were asked for");
+        }
+        void* rec = scratch_on(g.device);
+        g.adj_valid = false;
+        g.quasistatic(first_event, nevents, eps, direction, kicks, tol, niter_tol, max_iter, S, u_frame, f_frame, inc,
+                      rec);
     });
 }
 

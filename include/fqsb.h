@@ -260,6 +260,20 @@ int fqsb_batch_event_driven_step(fqsb_batch* b, double eps, int kick, int direct
 int fqsb_batch_quasistatic(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction, int kicks,
                            double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S, double* u_frame,
                            double* f_frame, int64_t* inc);
+/* fqsb_batch_quasistatic without any device allocation in the call. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and errors as fqsb_batch_quasistatic;
+ * the per-event records are staged in a fixed 8 MiB device array of the
+ * library (one per GPU, held until the process ends) instead of buffers
+ * allocated per call, so replicas x nevents is at most 262144 (a larger call
+ * fails with an error and changes nothing). Runs only where the whole
+ * protocol runs on the device (systems of one resident workgroup, inertial
+ * minimisation); other batches fail. Calls that overlap on one GPU share the
+ * array and race. */
+int fqsb_batch_quasistatic_noalloc(fqsb_batch* b, int64_t first_event, int64_t nevents, double eps, int direction,
+                                   int kicks, double tol, uint64_t niter_tol, uint64_t max_iter, int64_t* S,
+                                   double* u_frame, double* f_frame, int64_t* inc);
 /* The last fqsb_batch_quasistatic: time between HIP events on the launch
  * stream around its device work (ms), and the number of driver launches (0
  * if it was driven event by event from the host). */
