@@ -81,6 +81,7 @@ LR_N = 8192
 LR_BLOCK = 100
 
 RG_EPS = 1e-3
+RG_TUNE_MIN_PARTICLES = 262144  # groups tuned by the bench: at least this many particles
 QS_R = 1024
 QS_N = 1024
 QS_EPS = 1e-3
@@ -366,6 +367,76 @@ def run_longrange(args):
             "kernel_ms": product_ms,
         },
         "cpu_baseline": cpu,
+    }
+
+
+LRQ_R = 16
+LRQ_EVENTS = 5
+
+
+def run_lr_quasistatic(args):
+    """The reference's long-range protocol (examples/Line1d_Cuspy_Laplace_LongRange.py:
+    39-61: eventDrivenStep, then minimise) on the long-range configuration's
+    chain (N = 8192, alpha 1.5, without the random forcing, whose class has no
+    minimise): a batch of LRQ_R replicas with the forces of every minimise
+    step as one FP64 matrix-core product, and one system on the exact
+    sequential sum spread over tiles (the reference's API)."""
+    import torch
+
+    import frictionqpotspringblock_amd as fq
+    from frictionqpotspringblock_amd.batch import Batch
+
+    torch.cuda.init()
+    kw = dict(PARAMS, k_frame=1.0 / LR_N, shape=[LR_N], alpha=1.5)
+    b = Batch(LRQ_R, seed_stride=LR_N, potential="Cuspy", interactions="LongRange1d", **kw)
+    b.quasistatic(QS_EPS, 1)  # event 0
+    inc0 = b.inc.copy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q = b.quasistatic(QS_EPS, 2 * args.steps, first_event=1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps = b.inc - inc0
+    work = float(LR_N * steps.sum())
+    product_ms = b.longrange_product_ms
+    plain = None
+    if not args.no_plain:
+        s = fq.Line1d.System_Cuspy_LongRange(**kw)
+        s.u_frame = 0.0
+        s.minimise()
+        i0 = s.inc
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            s.eventDrivenStep(QS_EPS, False)
+            s.eventDrivenStep(QS_EPS, True)
+            s.minimise()
+        torch.cuda.synchronize()
+        plain = LR_N * (s.inc - i0) / (time.perf_counter() - t1)
+    return {
+        "metric": METRIC,
+        "value": work / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Weibull yield landscapes from pcg32 seeds r*{LR_N})",
+        "config": {
+            "workload": f"long-range quasistatic: {LRQ_R} Line1d Cuspy-LongRange(alpha=1.5) replicas x N={LR_N}, "
+            f"{args.steps} event pairs of the examples' protocol, minimise in matrix form",
+            "time_steps_per_replica_mean": float(steps.mean()),
+            "avalanche_size_mean": float(np.abs(q["S"][:, 1::2]).mean()),
+        },
+        "plain_value": plain,
+        "plain": "one System_Cuspy_LongRange through the reference's API (exact sum over tiles of 256 particles)",
+        "roofline": None,
+        "roofline_note": "minimise steps carry one matrix product each; see the longrange workload for k_lr_gemm",
+        "cpu_baseline": None,
     }
 
 
@@ -838,6 +909,14 @@ def run_ragged(args):
     rg = Ragged(systems)
     sizes = np.array(rg.sizes, dtype=float)
     smooth = np.array(["Smooth" in n for n, _ in systems])
+    tuned = {}
+    if not args.no_autotune:
+        # launch shape / block length of the larger groups, from trial events
+        # on a copy of the set (this set's events do not depend on it)
+        t0 = time.perf_counter()
+        res = rg.autotune(RG_EPS, events=1, smooth=smooth, min_particles=RG_TUNE_MIN_PARTICLES)
+        tuned = {r["key"]: {"path": r["path"], "choice": r["choice"]} for r in res.values()}
+        tuned["seconds"] = time.perf_counter() - t0
     nps = ragged_nopassing(systems)
     rgn = Ragged(nps)
     nsizes = np.array(rgn.sizes, dtype=float)
@@ -893,6 +972,7 @@ def run_ragged(args):
             "nopassing_systems": len(nps),
             "inertial_seconds": t_inertial,
             "particle_steps": work,
+            "autotune": tuned,
         },
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -1067,7 +1147,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument(
-        "--workload", choices=["flow", "longrange", "adjoint", "ragged", "quasistatic", "c0"], default="flow"
+        "--workload", choices=["flow", "longrange", "adjoint", "ragged", "quasistatic", "c0", "lrqs"],
+        default="flow"
     )
     ap.add_argument(
         "--steps", type=int, default=None,
@@ -1113,7 +1194,7 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"flow": 100, "longrange": 200, "adjoint": 5, "ragged": 50, "quasistatic": 100,
-                      "c0": 1}[args.workload]
+                      "c0": 1, "lrqs": 5}[args.workload]
     run = {
         "flow": run_flow,
         "longrange": run_longrange,
@@ -1121,6 +1202,7 @@ def main():
         "ragged": run_ragged,
         "quasistatic": run_quasistatic,
         "c0": run_c0,
+        "lrqs": run_lr_quasistatic,
     }[args.workload]
     line = run(args)
     if args.workload == "flow" and not args.no_extras:

@@ -241,6 +241,13 @@ SIGNATURES = {
     "fqsb_ragged_get_inc": (C.c_int, [_P, _ip]),
     "fqsb_ragged_last_steps": (C.c_int, [_P, _up]),
     "fqsb_ragged_chunk_index_at_align": (C.c_int, [_P, _ip]),
+    "fqsb_ragged_group_info": (C.c_int, [_P, C.c_uint64, _ip]),
+    "fqsb_ragged_group_systems": (C.c_int, [_P, C.c_uint64, _up]),
+    "fqsb_ragged_group_shapes": (
+        C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+    ),
+    "fqsb_ragged_set_group_launch": (C.c_int, [_P, C.c_uint64, C.c_int, C.c_int]),
+    "fqsb_ragged_set_group_block": (C.c_int, [_P, C.c_uint64, C.c_int]),
 }
 
 _lib = None

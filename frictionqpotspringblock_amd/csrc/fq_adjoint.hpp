@@ -15,11 +15,23 @@
 // to first: k_adj_record re-runs the window from its checkpoint (the same
 // expressions in the same order as the resident kernel, so the same
 // trajectory and the same wells) and writes a tape of u1, v and a of every
-// step and the well used by it; k_adj_reverse then walks the tape backwards,
-// loading the tape of each step while it finishes the one after it. One
-// workgroup per replica, PPT consecutive particles per thread, neighbours
-// exchanged through LDS; the tape is stored so that each of a wavefront's
-// tape loads and stores is contiguous (AdjBuffers).
+// step and the well used by it; k_adj_reverse then walks the tape backwards.
+// The tape is particle-major per step ([step][replica][particle]) and both
+// kernels give a thread the particles t, t + T, t + 2T, ... of its segment,
+// so that every tape load and store of a wavefront is contiguous; neighbours
+// are exchanged through LDS.
+//
+// k_adj_reverse spreads each replica over tiles (many workgroups per
+// replica): a tile walks its particles together with a halo of K + 1
+// particles on each side, whose cotangents start from the adjoint state of
+// their owners. The cotangent of a particle at the open end of the extended
+// segment lacks one neighbour, and the error this makes moves inwards by one
+// particle per reversed step (through the second difference of the
+// cotangents of the forces), so after the K steps of a window the tile's own
+// particles are exact. The adjoint state is double-buffered across windows
+// (tiles read their halo from the buffer that no tile writes in the same
+// launch); the parameter derivatives are summed per tile and then over the
+// tiles in a fixed order (k_adj_params).
 #pragma once
 
 #include "fq_physics.hpp"
@@ -27,16 +39,20 @@
 
 namespace fq {
 
+constexpr int ADJ_REV_THREADS = 512;
+constexpr int ADJ_REV_MAX_SLOTS = 8; // particles of the extended segment per thread (at most)
 
 // Record windows w0 + blockIdx.y (each of min(K, n - w K) steps) from their
 // start states, one workgroup per replica and window: every window is
-// independent given its checkpoint, so all of a group run at once.
-template <int PPT, int MAXT>
+// independent given its checkpoint, so all of a group run at once. Thread t
+// holds particles t + T k (k < SLOTS); the slips of a step are exchanged
+// through an LDS array of the whole chain, double-buffered by step parity
+// (one barrier per step).
+template <int SLOTS, int MAXT>
 __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__ views, AdjBuffers ab, int flow,
                                                      double v_frame)
 {
-    __shared__ double h_first[2][MAXT];
-    __shared__ double h_last[2][MAXT];
+    extern __shared__ double sx[]; // [2][N]
     const int r = blockIdx.x;
     const int64_t w = ab.w0 + blockIdx.y;
     const int steps = (int)(ab.n - w * ab.K < ab.K ? ab.n - w * ab.K : ab.K);
@@ -52,35 +68,31 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     ab.tj += blockIdx.y * ab.ts;
     ab.tuf += blockIdx.y * ab.K * ab.R;
     const SysView& S = views[r];
+    constexpr int T = MAXT; // launched with exactly MAXT threads
     const int t = threadIdx.x;
-    const int64_t N = ab.N;
+    const int N = (int)ab.N;
     const int64_t off = (int64_t)r * N;
-    const int64_t base = (int64_t)t * PPT;
-    const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
-    const int Tact = (int)((N + PPT - 1) / PPT);
     const Landscape& L = S.ls;
     const double* const Y = L.y;
     const int64_t nch = L.nchunk;
     const int32_t jmin = (int32_t)L.buffer;
     const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
-    // tape entry of register 0 of this thread in step 0 (register i: + 64 i)
     const int64_t Rnp = ab.R * ab.np;
-    const int64_t tq = (int64_t)r * ab.np + (int64_t)(t >> 6) * 64 * PPT + (t & 63);
 
-    double u[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT];
-    int32_t jl[PPT];
+    double u[SLOTS], v[SLOTS], a[SLOTS], yl[SLOTS], yr[SLOTS];
+    int32_t jl[SLOTS];
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        u[i] = v[i] = a[i] = yl[i] = yr[i] = 0.0;
-        jl[i] = 0;
-        if (i < cnt) {
-            const int64_t p = base + i;
-            u[i] = ab.wu[off + p];
-            v[i] = ab.wv[off + p];
-            a[i] = ab.wa[off + p];
-            jl[i] = ab.wj[off + p];
-            yl[i] = Y[p * nch + jl[i]];
-            yr[i] = Y[p * nch + jl[i] + 1];
+    for (int k = 0; k < SLOTS; ++k) {
+        u[k] = v[k] = a[k] = yl[k] = yr[k] = 0.0;
+        jl[k] = 0;
+        const int p = t + T * k;
+        if (p < N) {
+            u[k] = ab.wu[off + p];
+            v[k] = ab.wv[off + p];
+            a[k] = ab.wa[off + p];
+            jl[k] = ab.wj[off + p];
+            yl[k] = Y[p * nch + jl[k]];
+            yr[k] = Y[p * nch + jl[k] + 1];
         }
     }
     const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, inv_m = S.inv_m, kf = S.k_frame;
@@ -89,28 +101,32 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     int bad = 0;
 
     for (int n = 0; n < steps; ++n) {
-        const int b = n & 1;
+        double* const xb = sx + (n & 1) * N;
         if (flow) {
             uf = uf + dv;
         }
-        const int64_t tn = (int64_t)n * Rnp + tq;
+        const int64_t tn = (int64_t)n * Rnp + (int64_t)r * ab.np;
         if (t == 0) {
             ab.tuf[(int64_t)n * ab.R + r] = uf;
         }
+        double* const tv = ab.tv + tn + t;
+        double* const ta = ab.ta + tn + t;
+        double* const tu = ab.tu + tn + t;
+        int32_t* const tj = ab.tj + tn + t;
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                const int64_t p = base + i;
-                const int64_t q = tn + 64 * i;
-                ab.tv[q] = v[i];
-                ab.ta[q] = a[i];
-                u[i] = u[i] + dt * v[i] + c2 * a[i];
-                ab.tu[q] = u[i];
-                const double x = u[i];
-                if (!(yl[i] <= x && x < yr[i])) {
-                    const double* row = Y + p * nch;
-                    int32_t j = jl[i];
-                    double lo = yl[i], hi = yr[i];
+        for (int k = 0; k < SLOTS; ++k) {
+            const int p = t + T * k;
+            if (p < N) {
+                const int q = T * k; // compile-time offset from this thread's entries
+                tv[q] = v[k];
+                ta[q] = a[k];
+                u[k] = u[k] + dt * v[k] + c2 * a[k];
+                tu[q] = u[k];
+                const double x = u[k];
+                if (!(yl[k] <= x && x < yr[k])) {
+                    const double* row = Y + (int64_t)p * nch;
+                    int32_t j = jl[k];
+                    double lo = yl[k], hi = yr[k];
                     while (x >= hi && j < jmax) {
                         ++j;
                         lo = hi;
@@ -122,35 +138,26 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                         lo = row[j];
                     }
                     bad |= !(lo <= x && x < hi);
-                    jl[i] = j;
-                    yl[i] = lo;
-                    yr[i] = hi;
+                    jl[k] = j;
+                    yl[k] = lo;
+                    yr[k] = hi;
                 }
-                ab.tj[q] = jl[i];
-            }
-        }
-        if (cnt > 0) {
-            h_first[b][t] = u[0];
-#pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
-                    h_last[b][t] = u[i];
-                }
+                tj[q] = jl[k];
+                xb[p] = x;
             }
         }
         __syncthreads();
-        const double left = h_last[b][t == 0 ? Tact - 1 : t - 1];
-        const double right = h_first[b][t == Tact - 1 ? 0 : t + 1];
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                const double um = i == 0 ? left : u[i > 0 ? i - 1 : 0];
-                const double up = i == cnt - 1 ? right : u[i + 1 < PPT ? i + 1 : i];
-                const double fi = chain_force<INT_LAPLACE1D>(S, um, u[i], up);
-                const double fp = potential_force<POT_CUSPY>(S, u[i], yl[i], yr[i]);
-                const double ff = kf * (uf - u[i]);
+        for (int k = 0; k < SLOTS; ++k) {
+            const int p = t + T * k;
+            if (p < N) {
+                const double um = xb[p == 0 ? N - 1 : p - 1];
+                const double up = xb[p == N - 1 ? 0 : p + 1];
+                const double fi = chain_force<INT_LAPLACE1D>(S, um, u[k], up);
+                const double fp = potential_force<POT_CUSPY>(S, u[k], yl[k], yr[k]);
+                const double ff = kf * (uf - u[k]);
                 const double sum3 = ff + fp + fi;
-                const double vn = v[i], an = a[i];
+                const double vn = v[k], an = a[k];
                 double vv = vn + dt * an;
                 double f = sum3 + meta * vv;
                 double aa = f * inv_m;
@@ -160,8 +167,8 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                 vv = vn + hdt * (an + aa);
                 f = sum3 + meta * vv;
                 aa = f * inv_m;
-                v[i] = vv;
-                a[i] = aa;
+                v[k] = vv;
+                a[k] = aa;
             }
         }
     }
@@ -170,140 +177,118 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     }
 }
 
-// Walk the tape of one window backwards, updating the adjoint state and the
-// parameter / yield-position derivatives. Since dg/dy_l = dg/dy_r = mu/2, one
-// accumulator per particle collects the derivative for both yield positions
-// of its current well; it is added to the output when the well changes.
-template <int PPT, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict__ views, AdjBuffers ab, int steps)
+// Walk the tape of one window backwards on one tile of one replica
+// (blockIdx.x = replica * nt + tile), updating the adjoint state of the
+// tile's particles and its partial parameter derivatives. Since dg/dy_l =
+// dg/dy_r = mu/2, one accumulator per particle collects the derivative for
+// both yield positions of its current well; it is added to the output when
+// the well changes.
+template <int SLOTS>
+__global__ void __launch_bounds__(ADJ_REV_THREADS) k_adj_reverse(const SysView* __restrict__ views, AdjBuffers ab,
+                                                                   int steps)
 {
-    // neighbour exchange: slips u1 (h_*) and cotangents of g (g_*) of a step;
+    // neighbour exchange: slips u1 (su) and cotangents of g (sg) of a step;
     // two barriers per step (a buffer is rewritten only after the barrier
     // that follows every read of it)
-    __shared__ double h_first[MAXT];
-    __shared__ double h_last[MAXT];
-    __shared__ double g_first[MAXT];
-    __shared__ double g_last[MAXT];
-    __shared__ double s_red[4][MAXT / 64];
-    const int r = blockIdx.x;
+    __shared__ double su[ADJ_REV_THREADS * SLOTS];
+    __shared__ double sg[ADJ_REV_THREADS * SLOTS];
+    __shared__ double s_red[4][ADJ_REV_THREADS / 64];
+    const int r = blockIdx.x / ab.nt, tile = blockIdx.x % ab.nt;
     const SysView& S = views[r];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
-    const int64_t N = ab.N;
+    constexpr int T = ADJ_REV_THREADS;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int nw = T / 64;
+    const int N = (int)ab.N;
     const int64_t off = (int64_t)r * N;
-    const int64_t base = (int64_t)t * PPT;
-    const int cnt = (int)(N - base >= PPT ? PPT : (N - base > 0 ? N - base : 0));
-    const int Tact = (int)((N + PPT - 1) / PPT);
+    const bool ring = ab.nt == 1; // one tile: the whole periodic chain, no halo
+    const int p0 = tile * (int)ab.tp;
+    const int p1 = p0 + (int)ab.tp < N ? p0 + (int)ab.tp : N;
+    const int H = ring ? 0 : (int)ab.halo;
+    const int E = ring ? N : (p1 - p0) + 2 * H; // particles of the extended segment
     const double* const Y = S.ls.y;
     const int64_t nch = S.ls.nchunk;
     const int64_t Rnp = ab.R * ab.np;
-    const int64_t tq = (int64_t)r * ab.np + (int64_t)(t >> 6) * 64 * PPT + (t & 63);
 
     const double dt = S.dt, c2 = 0.5 * dt * dt, hdt = 0.5 * dt, meta = -S.eta, w = S.inv_m, kf = S.k_frame;
     const double mu = S.mu, k = S.k;
     const double mweta = meta * w; // -eta / m
     const double hmu = 0.5 * mu;
 
+    // slot i: local index e = t + T i of the extended segment, particle p
+    int pp[SLOTS];
+    bool own[SLOTS], live[SLOTS];
+#pragma unroll
+    for (int i = 0; i < SLOTS; ++i) {
+        const int e = t + T * i;
+        live[i] = e < E;
+        own[i] = live[i] && (ring || (e >= H && e < H + (p1 - p0)));
+        int q = ring ? e : p0 - H + e;
+        q = q < 0 ? q + N : (q >= N ? q - N : q);
+        pp[i] = q;
+    }
     // U, V, A: cotangents of u, v, a after the step being reversed
     // jc, ymc: the well of the current tape step and its mid-point (read from
     // the chunk only when the well changes)
-    double U[PPT], V[PPT], A[PPT], acc[PPT], ymc[PPT];
-    int32_t jc[PPT];
+    double U[SLOTS], V[SLOTS], A[SLOTS], acc[SLOTS], ymc[SLOTS];
+    int32_t jc[SLOTS];
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
+    for (int i = 0; i < SLOTS; ++i) {
         U[i] = V[i] = A[i] = acc[i] = ymc[i] = 0.0;
         jc[i] = -1;
-        if (i < cnt) {
-            const int64_t p = base + i;
-            U[i] = ab.au[off + p];
-            V[i] = ab.av[off + p];
-            A[i] = ab.aa[off + p];
+        if (live[i]) {
+            U[i] = ab.au[off + pp[i]];
+            V[i] = ab.av[off + pp[i]];
+            A[i] = ab.aa[off + pp[i]];
         }
     }
     double gmu = 0.0, gk = 0.0, gkf = 0.0, geta = 0.0;
 
-    // the tape of the step being reversed (u1, v, a, well, frame position),
-    // loaded while the step before it in the walk is finished: u1 as soon as
-    // it has been read (at the top of the step), the rest once the first
-    // half of the step has used it
-    double pu[PPT], pv[PPT], pa[PPT];
-    int32_t pj[PPT];
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        pu[i] = pv[i] = pa[i] = 0.0;
-        pj[i] = 0;
-    }
-    double puf = 0.0;
-    auto load_u1 = [&](int n) {
-        const int64_t tn = (int64_t)n * Rnp + tq;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                pu[i] = ab.tu[tn + 64 * i];
-            }
-        }
-    };
-    auto load_rest = [&](int n) {
-        const int64_t tn = (int64_t)n * Rnp + tq;
-#pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                const int64_t q = tn + 64 * i;
-                pv[i] = ab.tv[q];
-                pa[i] = ab.ta[q];
-                pj[i] = ab.tj[q];
-            }
-        }
-        puf = ab.tuf[(int64_t)n * ab.R + r];
-    };
-    if (steps > 0) {
-        load_u1(steps - 1);
-        load_rest(steps - 1);
-    }
-    // every load above complete: the loop's wait counts then concern only
-    // the tape loads of the step before, which stay in flight until the top
-    // of the next iteration
-    __builtin_amdgcn_s_waitcnt(0);
-
     auto flush = [&](int i) {
-        if (ab.gy && jc[i] >= 0) {
-            double* row = ab.gy + (off + base + i) * nch;
+        if (ab.gy && own[i] && jc[i] >= 0) {
+            double* row = ab.gy + (off + pp[i]) * nch;
             row[jc[i]] += acc[i];
             row[jc[i] + 1] += acc[i];
         }
         acc[i] = 0.0;
     };
+    // neighbour of local index e at offset d (+1 or -1): periodic on the
+    // ring; at the open ends of a tile's segment, any finite value (the
+    // error zone that never reaches the tile's own particles)
+    auto nb = [&](const double* s, int e, int d, double self) -> double {
+        int q = e + d;
+        if (ring) {
+            q = q < 0 ? q + E : (q >= E ? q - E : q);
+            return s[q];
+        }
+        return (q < 0 || q >= E) ? self : s[q];
+    };
 
     for (int n = steps - 1; n >= 0; --n) {
-        const double uf = puf;
-        double x[PPT]; // u1 of the step, then the cotangent of g
+        const int64_t tn = (int64_t)n * Rnp + (int64_t)r * ab.np;
+        const double uf = ab.tuf[(int64_t)n * ab.R + r];
+        double x[SLOTS], pv[SLOTS], pa[SLOTS];
+        int32_t pj[SLOTS];
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            x[i] = pu[i];
-        }
-        if (cnt > 0) {
-            h_first[t] = x[0];
-#pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
-                    h_last[t] = x[i];
-                }
+        for (int i = 0; i < SLOTS; ++i) {
+            x[i] = pv[i] = pa[i] = 0.0;
+            pj[i] = 0;
+            if (live[i]) {
+                const int64_t q = tn + pp[i];
+                x[i] = ab.tu[q];
+                pv[i] = ab.tv[q];
+                pa[i] = ab.ta[q];
+                pj[i] = ab.tj[q];
+                su[t + T * i] = x[i];
             }
         }
         __syncthreads();
-        const double left = h_last[t == 0 ? Tact - 1 : t - 1];
-        const double right = h_first[t == Tact - 1 ? 0 : t + 1];
-        if (n > 0) {
-            load_u1(n - 1);
-        }
-        double xm = left; // u1 of the particle before i (its original value)
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
-                const int64_t p = base + i;
+        for (int i = 0; i < SLOTS; ++i) {
+            if (live[i]) {
+                const int e = t + T * i;
+                const int64_t p = pp[i];
                 const double u1 = x[i];
-                const double up = i == cnt - 1 ? right : x[i + 1 < PPT ? i + 1 : i];
-                const double lu = xm - 2 * u1 + up;
-                xm = u1;
+                const double lu = nb(su, e, -1, u1) - 2 * u1 + nb(su, e, +1, u1);
                 const int32_t j = pj[i];
                 if (j != jc[i]) {
                     flush(i);
@@ -322,56 +307,44 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
                 // a3 = w (g - eta v3)
                 double G = w * A[i];
                 const double V3 = V[i] + mweta * A[i];
-                geta -= w * v3 * A[i];
+                double ge = -(w * v3 * A[i]);
                 // v3 = v + hdt a + hdt a2
                 double vb = V3, ab_ = hdt * V3;
                 const double A2 = hdt * V3;
                 G += w * A2;
                 const double V2 = mweta * A2;
-                geta -= w * v2 * A2;
+                ge -= w * v2 * A2;
                 // v2 = v + hdt a + hdt a1
                 vb += V2;
                 ab_ += hdt * V2;
                 const double A1 = hdt * V2;
                 G += w * A1;
                 const double V1 = mweta * A1;
-                geta -= w * v1 * A1;
+                ge -= w * v1 * A1;
                 // v1 = v + dt a
                 vb += V1;
                 ab_ += dt * V1;
-                gkf += (uf - u1) * G;
-                gmu += (ym - u1) * G;
-                gk += lu * G;
-                acc[i] += hmu * G;
-                V[i] = vb;  // partial cotangents of v, a (the u1 part follows)
+                if (own[i]) {
+                    geta += ge;
+                    gkf += (uf - u1) * G;
+                    gmu += (ym - u1) * G;
+                    gk += lu * G;
+                    acc[i] += hmu * G;
+                }
+                V[i] = vb; // partial cotangents of v, a (the u1 part follows)
                 A[i] = ab_;
                 x[i] = G;
-            }
-        }
-        if (n > 0) {
-            load_rest(n - 1);
-        }
-        if (cnt > 0) {
-            g_first[t] = x[0];
-#pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                if (i == cnt - 1) { // (an indexed read would move the array to scratch)
-                    g_last[t] = x[i];
-                }
+                sg[e] = G;
             }
         }
         __syncthreads();
-        const double gleft = g_last[t == 0 ? Tact - 1 : t - 1];
-        const double gright = g_first[t == Tact - 1 ? 0 : t + 1];
-        double gm = gleft;
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i < cnt) {
+        for (int i = 0; i < SLOTS; ++i) {
+            if (live[i]) {
+                const int e = t + T * i;
                 const double G = x[i];
-                const double gp = i == cnt - 1 ? gright : x[i + 1 < PPT ? i + 1 : i];
                 // g depends on u1 through -(k_frame + mu) u1 + k L u1 (L symmetric)
-                const double U1 = U[i] - (kf + mu) * G + k * (gm - 2 * G + gp);
-                gm = G;
+                const double U1 = U[i] - (kf + mu) * G + k * (nb(sg, e, -1, G) - 2 * G + nb(sg, e, +1, G));
                 // u1 = u + dt v + c2 a
                 U[i] = U1;
                 V[i] = V[i] + dt * U1;
@@ -380,16 +353,15 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
         }
     }
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-        if (i < cnt) {
-            const int64_t p = base + i;
-            ab.au[off + p] = U[i];
-            ab.av[off + p] = V[i];
-            ab.aa[off + p] = A[i];
+    for (int i = 0; i < SLOTS; ++i) {
+        if (own[i]) {
+            ab.bu[off + pp[i]] = U[i];
+            ab.bv[off + pp[i]] = V[i];
+            ab.ba[off + pp[i]] = A[i];
             flush(i);
         }
     }
-    // parameter derivatives: block sums in a fixed order, added to the totals
+    // parameter derivatives of the tile: block sums in a fixed order
     gmu = wave_sum(gmu);
     gk = wave_sum(gk);
     gkf = wave_sum(gkf);
@@ -406,8 +378,25 @@ __global__ void __launch_bounds__(MAXT) k_adj_reverse(const SysView* __restrict_
         for (int q = 0; q < nw; ++q) {
             s += s_red[t][q];
         }
-        ab.gpar[(int64_t)r * 4 + t] += s;
+        ab.gpart[(int64_t)blockIdx.x * 4 + t] = s;
     }
+}
+
+// The parameter derivatives of a window: the tiles' partial sums of each
+// replica, in tile order, added to the totals (one thread per replica and
+// parameter).
+__global__ void __launch_bounds__(256) k_adj_params(AdjBuffers ab)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ab.R * 4) {
+        return;
+    }
+    const int64_t r = i / 4, c = i % 4;
+    double s = 0.0;
+    for (int tile = 0; tile < ab.nt; ++tile) {
+        s += ab.gpart[(r * ab.nt + tile) * 4 + c];
+    }
+    ab.gpar[i] += s;
 }
 
 } // namespace fq

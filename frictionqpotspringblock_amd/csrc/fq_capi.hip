@@ -153,6 +153,8 @@ struct Group {
     int32_t* tp_j = nullptr;
     double* ck_uf_d = nullptr; // device copy of ck_uf
     double *ad_u = nullptr, *ad_v = nullptr, *ad_a = nullptr, *ad_par = nullptr;
+    double *ad_u2 = nullptr, *ad_v2 = nullptr, *ad_a2 = nullptr, *ad_gpart = nullptr;
+    size_t ad_gpart_cap = 0;
     int32_t* ad_err = nullptr;
     int64_t ck_cap = 0;  // windows the checkpoint buffers hold
     size_t tp_cap = 0;   // entries each tape buffer holds
@@ -416,6 +418,33 @@ struct Group {
         threads = best_thr;
     }
 
+    // Resident launch shapes (particles per thread, threads) that fit the
+    // group: for each kernel variant, the fewest threads that hold the largest
+    // system, and twice that if the variant allows it.
+    std::vector<std::pair<int, int>> launch_shapes() const
+    {
+        std::vector<std::pair<int, int>> out;
+        if (streaming || minim == FQSB_MIN_OVERDAMPED) {
+            return out;
+        }
+        int vp[64], vm[64];
+        const int nv = std::min(fq::resident_variants(pot, inter, ext, vp, vm, 64), 64);
+        for (int i = 0; i < nv; ++i) {
+            const int p = vp[i], mt = vm[i];
+            const int thr = (int)((((N + p - 1) / p) + 63) / 64 * 64);
+            for (int t2 : {thr, 2 * thr}) {
+                if (t2 > mt || t2 > 1024 || fq::resident_lds(inter, N, t2) > lds_limit) {
+                    continue;
+                }
+                const std::pair<int, int> sh(p, t2);
+                if (std::find(out.begin(), out.end(), sh) == out.end()) {
+                    out.push_back(sh);
+                }
+            }
+        }
+        return out;
+    }
+
     void launch_resident(const fq::RunArgs& ra)
     {
         if (minim == FQSB_MIN_OVERDAMPED) {
@@ -666,7 +695,7 @@ struct Group {
     void free_adjoint()
     {
         for (double** q : {&ck_u, &ck_v, &ck_a, &tp_u, &tp_v, &tp_a, &tp_uf, &ck_uf_d, &ad_u, &ad_v, &ad_a,
-                           &ad_par}) {
+                           &ad_par, &ad_u2, &ad_v2, &ad_a2, &ad_gpart}) {
             dfree(*q);
             *q = nullptr;
         }
@@ -676,6 +705,7 @@ struct Group {
         }
         ck_cap = 0;
         tp_cap = 0;
+        ad_gpart_cap = 0;
         adj_valid = false;
         for (auto& e : eva) {
             if (e) {
@@ -775,17 +805,13 @@ struct Group {
         }
         adjoint_check();
         const size_t RN = (size_t)total;
-        // launch shape: fewest particles per thread that fits, except that
-        // chains above 3072 particles take 8 per thread on 512 threads (at
-        // N = 4096, k_adj_reverse takes 0.51 ms per 100-step window of 64
-        // replicas this way, 0.59 ms with 4 per thread on 1024 threads)
+        // launch shape of the record kernel: fewest particles per thread that
+        // fits (4 x 1024 at N = 4096 holds its registers without spills; 8 x
+        // 512 does not)
         int vp[8], vm[8];
         const int nv = fq::adj_variants(vp, vm, 8);
         int ppt = 0, threads = 0;
         for (int i = 0; i < nv; ++i) {
-            if ((vp[i] == 8) != (N > 3072)) {
-                continue;
-            }
             const int64_t need = (N + vp[i] - 1) / vp[i];
             const int thr = (int)(((need + 63) / 64) * 64);
             if (thr <= vm[i]) {
@@ -797,10 +823,33 @@ struct Group {
         if (!ppt) {
             throw Error("no adjoint kernel for this system size");
         }
-        // tape entries per step (each replica's block padded to whole
-        // wavefronts of the launch shape)
-        const int64_t np = fq::adj_tape_stride(N, ppt);
+        // tape entries per step: particle-major, N per replica
+        const int64_t np = N;
         const size_t RNt = (size_t)(R * np);
+        // tiles of the reverse pass: enough workgroups to fill the chip (at
+        // least 256), each tile's particles at least four times its halo of
+        // K + 1 particles per side (one tile: the whole periodic chain, no
+        // halo); FQSB_ADJ_TILES=n asks for n tiles per replica
+        const int64_t halo = adj_K + 1;
+        int64_t nt = 1;
+        if (const char* e = std::getenv("FQSB_ADJ_TILES")) {
+            nt = std::max<int64_t>(1, std::atoll(e));
+        }
+        else {
+            while (R * nt < 256 && (N + 2 * nt - 1) / (2 * nt) >= 4 * halo) {
+                nt *= 2;
+            }
+        }
+        int64_t tp = (N + nt - 1) / nt;
+        nt = (N + tp - 1) / tp;
+        if (nt > 1 && tp < halo) {
+            nt = 1;
+            tp = N;
+        }
+        const int slots = fq::adj_reverse_slots(nt == 1 ? N : tp + 2 * halo);
+        if (slots == 0) {
+            throw Error("the adjoint's reverse tiles do not fit: window too long for the chain");
+        }
         // the tape of a group of windows, recorded by one launch (every window
         // starts from its checkpoint): as many windows as fit in ADJ_TAPE_BYTES
         const size_t per_window = RNt * (size_t)adj_K * (3 * sizeof(double) + sizeof(int32_t)) +
@@ -824,8 +873,16 @@ struct Group {
             ad_u = dalloc<double>(RN);
             ad_v = dalloc<double>(RN);
             ad_a = dalloc<double>(RN);
+            ad_u2 = dalloc<double>(RN);
+            ad_v2 = dalloc<double>(RN);
+            ad_a2 = dalloc<double>(RN);
             ad_par = dalloc<double>((size_t)R * 4);
             ad_err = dalloc<int32_t>(1);
+        }
+        if ((size_t)(R * nt * 4) > ad_gpart_cap) {
+            dfree(ad_gpart);
+            ad_gpart = dalloc<double>((size_t)(R * nt * 4));
+            ad_gpart_cap = (size_t)(R * nt * 4);
         }
         dfree(ck_uf_d);
         ck_uf_d = dalloc<double>(ck_uf.size());
@@ -872,11 +929,25 @@ struct Group {
                 abw.ta = tp_a + to * RNt;
                 abw.tj = tp_j + to * RNt;
                 abw.tuf = tp_uf + to * (size_t)R;
+                // adjoint state: read from one buffer, written to the other
+                abw.au = ad_u;
+                abw.av = ad_v;
+                abw.aa = ad_a;
+                abw.bu = ad_u2;
+                abw.bv = ad_v2;
+                abw.ba = ad_a2;
+                abw.gpart = ad_gpart;
+                abw.nt = (int32_t)nt;
+                abw.tp = tp;
+                abw.halo = halo;
                 const int nsteps = (int)std::min<int64_t>(adj_K, adj_n - w * adj_K);
-                if (!fq::launch_adj_reverse(ppt, threads, views, abw, nsteps, stream)) {
+                if (!fq::launch_adj_reverse(views, abw, slots, nsteps, stream)) {
                     throw Error("adjoint reverse kernel not available");
                 }
                 HIPC(hipGetLastError());
+                std::swap(ad_u, ad_u2);
+                std::swap(ad_v, ad_v2);
+                std::swap(ad_a, ad_a2);
             }
             HIPC(hipEventRecord(eva[2], stream));
             HIPC(hipEventSynchronize(eva[2]));
@@ -1792,7 +1863,9 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
                 g->h2d(g->tb_tiles, tb.data(), tb.size());
                 g->tb_first = dalloc<int32_t>((size_t)R + 1);
                 g->h2d(g->tb_first, tbf.data(), (size_t)R + 1);
-                g->tile_part_k = dalloc<double>(2 * (size_t)g->tile_k * tb.size());
+                // room for blocks of up to TB_KMAX_STEPS (the block length can be
+                // changed later, fqsb_ragged_set_group_block)
+                g->tile_part_k = dalloc<double>(2 * (size_t)fq::TB_KMAX_STEPS * tb.size());
                 g->tile_fail = dalloc<int32_t>(tb.size());
             }
             g->tile_flag = dalloc<int32_t>((size_t)R);
@@ -3196,6 +3269,93 @@ int fqsb_ragged_sizes(const fqsb_ragged* r, uint64_t* out)
         out[i] = (uint64_t)r->sizes[i];
     }
     return 0;
+}
+
+int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        const Group& g = *r->groups[(size_t)group];
+        int64_t path = FQSB_PATH_RESIDENT;
+        if (g.np_tiled) {
+            path = FQSB_PATH_NOPASSING_TILES;
+        }
+        else if (g.minim == FQSB_MIN_OVERDAMPED) {
+            path = FQSB_PATH_NOPASSING_RESIDENT;
+        }
+        else if (g.tiled) {
+            path = FQSB_PATH_TILES;
+        }
+        else if (g.streaming) {
+            path = FQSB_PATH_STREAMING;
+        }
+        const int64_t v[12] = {g.R, g.N, path, g.ppt, g.threads, g.tile_k, (int64_t)g.d.rank, g.pot, g.inter, g.ext,
+                               g.minim, g.total};
+        std::copy(v, v + 12, info);
+    });
+}
+
+int fqsb_ragged_group_systems(const fqsb_ragged* r, uint64_t group, uint64_t* idx)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        for (size_t i = 0; i < r->where.size(); ++i) {
+            if (r->where[i].first == group) {
+                idx[r->where[i].second] = (uint64_t)i;
+            }
+        }
+    });
+}
+
+int fqsb_ragged_group_shapes(const fqsb_ragged* r, uint64_t group, int* ppt, int* threads, int cap, int* count)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        const auto sh = r->groups[(size_t)group]->launch_shapes();
+        const int n = (int)std::min<size_t>(sh.size(), (size_t)std::max(cap, 0));
+        for (int i = 0; i < n; ++i) {
+            ppt[i] = sh[(size_t)i].first;
+            threads[i] = sh[(size_t)i].second;
+        }
+        *count = (int)sh.size();
+    });
+}
+
+int fqsb_ragged_set_group_launch(fqsb_ragged* r, uint64_t group, int ppt, int threads)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        Group& g = *r->groups[(size_t)group];
+        if (g.streaming || g.minim == FQSB_MIN_OVERDAMPED) {
+            throw Error("the group does not run on the resident kernels");
+        }
+        g.choose_launch(ppt, threads);
+    });
+}
+
+int fqsb_ragged_set_group_block(fqsb_ragged* r, uint64_t group, int steps)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        Group& g = *r->groups[(size_t)group];
+        if (!g.tiled || g.tile_part_k == nullptr) {
+            throw Error("the group does not run blocks of steps over tiles");
+        }
+        if (steps < 1 || steps > fq::TB_KMAX_STEPS) {
+            throw Error("the block length must be 1 to " + std::to_string(fq::TB_KMAX_STEPS) + " steps");
+        }
+        g.tile_k = steps;
+    });
 }
 
 int fqsb_ragged_time_steps(fqsb_ragged* r, uint64_t n)

@@ -186,11 +186,8 @@ struct LrBuffers {
 };
 
 // Buffers of the adjoint of a batch (fq_adjoint.hpp); [R*N] arrays are
-// replica after replica. The tape holds, per step, R blocks of np entries
-// (np = N rounded up to 64 PPT): particle p = 64 PPT w + PPT l + i (wavefront
-// w, lane l, register i of the kernels' launch shape) is entry
-// 64 PPT w + 64 i + l of its replica's block, so that a wavefront's load of
-// one register is contiguous.
+// replica after replica. The tape holds, per step, R blocks of np = N entries
+// (particle-major).
 struct AdjBuffers {
     double* tu;     // tape [K][R*np]: u1 = u + dt v + dt^2/2 a of step n
     double* tv;     // tape: v at the start of step n
@@ -202,10 +199,14 @@ struct AdjBuffers {
     double* wa;
     int32_t* wj;
     double* wuf;    // [R] frame position at the window start
-    double* au;     // adjoint state [R*N] (cotangent of u, v, a)
+    double* au;     // adjoint state [R*N] (cotangent of u, v, a) after the window
     double* av;
     double* aa;
+    double* bu;     // adjoint state [R*N] before the window (written by k_adj_reverse)
+    double* bv;
+    double* ba;
     double* gpar;   // [R][4] accumulated d/d(mu, k_interactions, k_frame, eta)
+    double* gpart;  // [R][nt][4] the window's partial sums per tile
     double* gy;     // [R][N][nchunk] accumulated d/dy, or null
     int32_t* err;   // set to 1 if a well walk left the landscape window
     int64_t N;
@@ -218,13 +219,11 @@ struct AdjBuffers {
     int64_t K, n;
     int64_t w0;
     int64_t np; // tape entries per replica and step
+    // k_adj_reverse: tiles per replica, particles per tile, halo width
+    int32_t nt;
+    int32_t pad1;
+    int64_t tp, halo;
 };
-
-// tape entries per replica and step for PPT particles per thread
-FQ_HD constexpr int64_t adj_tape_stride(int64_t N, int ppt)
-{
-    return (N + 64 * ppt - 1) / (64 * ppt) * (64 * ppt);
-}
 
 struct RunArgs {
     int32_t mode;

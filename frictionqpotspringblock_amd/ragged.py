@@ -60,6 +60,7 @@ class Ragged:
         h = C.c_void_p()
         _check(capi().fqsb_ragged_create(arr, n, C.byref(h)))
         self._h = h
+        self._systems = [(name, dict(kw)) for name, kw in systems]
         self._n = n
         self._sizes = [int(np.prod(s)) for s in self._shapes]
         self._offsets = np.concatenate([[0], np.cumsum(self._sizes)]).astype(int)
@@ -129,6 +130,109 @@ class Ragged:
             lambda *a: capi().fqsb_ragged_quasistatic(self._h, *a), self._n, eps, nevents, first_event, direction,
             kicks_only, tol, niter_tol, max_iter, masked=True, mask=self._mask(active),
         )
+
+    # launch groups and their tuning ---------------------------------------
+
+    PATHS = {0: "resident", 1: "tiles", 2: "streaming", 3: "nopassing-resident", 4: "nopassing-tiles"}
+
+    def group_info(self, g):
+        """Launch group g: its systems (indices), largest size, path and
+        current launch shape / block length."""
+        info = np.empty(12, dtype=np.int64)
+        _check(capi().fqsb_ragged_group_info(self._h, int(g), info.ctypes.data_as(_ip)))
+        idx = np.empty(int(info[0]), dtype=np.uint64)
+        _check(capi().fqsb_ragged_group_systems(self._h, int(g), idx.ctypes.data_as(_up)))
+        ppt = (C.c_int * 32)()
+        thr = (C.c_int * 32)()
+        cnt = C.c_int()
+        _check(capi().fqsb_ragged_group_shapes(self._h, int(g), ppt, thr, 32, C.byref(cnt)))
+        return {
+            "systems": idx.astype(int),
+            "N": int(info[1]),
+            "particles": int(info[11]),
+            "path": self.PATHS.get(int(info[2]), str(info[2])),
+            "launch": (int(info[3]), int(info[4])),
+            "block": int(info[5]),
+            "key": "-".join(str(int(x)) for x in info[6:11]) + f"-{int(np.ceil(np.log2(max(int(info[1]), 1))))}",
+            "shapes": [(int(ppt[i]), int(thr[i])) for i in range(min(cnt.value, 32))],
+        }
+
+    def set_group_launch(self, g, ppt, threads):
+        _check(capi().fqsb_ragged_set_group_launch(self._h, int(g), int(ppt), int(threads)))
+
+    def set_group_block(self, g, steps):
+        _check(capi().fqsb_ragged_set_group_block(self._h, int(g), int(steps)))
+
+    def autotune(self, eps=1e-3, events=2, blocks=(16, 32, 64, 128), cache=None, smooth=None, min_particles=0):
+        """Choose each launch group's launch shape (resident groups) or block
+        length (tiled groups) by timing a few events of the examples'
+        quasistatic protocol per candidate, on a copy of this set built from
+        the same descriptions (so this set's own state is left as it is);
+        results do not depend on the choice. `smooth` marks the systems that
+        take kicks only (their maxUniformDisplacement raises). With `cache` (a
+        JSON file), choices found there for a group's key (class and size
+        class) are applied without trials and new ones are added. Groups of
+        fewer than `min_particles` particles keep their default. Returns
+        {group: {"key", "path", "choice", "trials_ms"}}."""
+        import json
+        import os
+        import time
+
+        known = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+        smooth = np.zeros(self._n, dtype=bool) if smooth is None else np.asarray(smooth, dtype=bool)
+        out = {}
+        trial = None
+        first = 1
+        for g in range(self.groups):
+            info = self.group_info(g)
+            key = info["key"]
+            if info["particles"] < min_particles:
+                continue
+            if info["path"] == "resident" and len(info["shapes"]) > 1:
+                cands = [("launch", sh) for sh in info["shapes"]]
+            elif info["path"] == "tiles" and info["block"] > 0:
+                cands = [("block", k) for k in blocks]
+            else:
+                continue
+            if key in known:
+                kind, val = known[key]["choice"]
+                self._apply(g, kind, val)
+                out[g] = {"key": key, "path": info["path"], "choice": [kind, val], "trials_ms": None, "cached": True}
+                continue
+            if trial is None:
+                trial = Ragged(self._systems)
+                trial.quasistatic(eps, 1)  # event 0 of every system
+            act = np.zeros(self._n, dtype=bool)
+            act[info["systems"]] = True
+            times = {}
+            for kind, val in cands:
+                trial._apply(g, kind, val)
+                t0 = time.perf_counter()
+                if np.any(act & ~smooth):
+                    trial.quasistatic(eps, 2 * events, first_event=first, active=act & ~smooth)
+                if np.any(act & smooth):
+                    trial.quasistatic(eps, events, first_event=(first + 1) // 2, kicks_only=True, active=act & smooth)
+                times[f"{kind}:{val}"] = (time.perf_counter() - t0) * 1e3
+                first += 2 * events
+            best = min(times, key=times.get)
+            kind, val = best.split(":", 1)
+            val = [int(x) for x in val.strip("()").split(",")] if kind == "launch" else int(val)
+            self._apply(g, kind, val)
+            known[key] = {"choice": [kind, val]}
+            out[g] = {"key": key, "path": info["path"], "choice": [kind, val], "trials_ms": times, "cached": False}
+        if cache:
+            with open(cache, "w") as f:
+                json.dump(known, f, indent=1, sort_keys=True)
+        return out
+
+    def _apply(self, g, kind, val):
+        if kind == "launch":
+            self.set_group_launch(g, *val)
+        else:
+            self.set_group_block(g, val)
 
     def _split(self, flat):
         return [flat[a:b].reshape(s) for a, b, s in zip(self._offsets[:-1], self._offsets[1:], self._shapes)]

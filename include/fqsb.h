@@ -407,6 +407,28 @@ int fqsb_ragged_get_inc(fqsb_ragged* r, int64_t* out);
 int fqsb_ragged_last_steps(fqsb_ragged* r, uint64_t* out);
 int fqsb_ragged_chunk_index_at_align(fqsb_ragged* r, int64_t* out);
 
+/* Launch groups of a ragged set, for tuning (no reference counterpart). The
+ * path a group runs on: */
+enum {
+    FQSB_PATH_RESIDENT = 0,            /* one workgroup per system, many steps per launch */
+    FQSB_PATH_TILES = 1,               /* tiles of a system over many workgroups (blocks of steps) */
+    FQSB_PATH_STREAMING = 2,           /* one workgroup per system, state in HBM */
+    FQSB_PATH_NOPASSING_RESIDENT = 3,  /* no-passing sweeps, one workgroup per system */
+    FQSB_PATH_NOPASSING_TILES = 4      /* no-passing sweeps over tiles */
+};
+/* info[12] = systems, largest N, path, particles per thread, threads, block
+ * length of the tiled path, rank, potential, interactions, external,
+ * minimisation, particles. */
+int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info);
+/* Indices (in the order of descs) of the group's systems: info[0] values. */
+int fqsb_ragged_group_systems(const fqsb_ragged* r, uint64_t group, uint64_t* idx);
+/* Resident launch shapes the group can use (at most cap; *count: all). */
+int fqsb_ragged_group_shapes(const fqsb_ragged* r, uint64_t group, int* ppt, int* threads, int cap, int* count);
+/* Set a resident group's launch shape, or a tiled group's block length
+ * (1 .. 128 steps per launch). Results do not depend on either. */
+int fqsb_ragged_set_group_launch(fqsb_ragged* r, uint64_t group, int ppt, int threads);
+int fqsb_ragged_set_group_block(fqsb_ragged* r, uint64_t group, int steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,3 +171,36 @@ def test_ragged_masks_and_errors(oracle):
         assert rgn.minimise()[0] == o.minimise() == 0
         _assert_same(rgn, [o], f"no-passing {n}")
         assert rgn.last_steps[0] > 10
+
+
+def test_ragged_autotune(oracle, tmp_path):
+    """The tuner picks a launch shape per resident group and a block length
+    per tiled group by timing trial events on a copy of the set; the choice
+    changes no result (a tuned and an untuned set stay bit-identical), and a
+    cache file applies the same choices without trials."""
+    from frictionqpotspringblock_amd.ragged import Ragged
+
+    systems = _systems() + [("System_Cuspy_Laplace", _par(5000, 12))]
+    systems = [x for x in systems if "Nopassing" not in x[0]]
+    smooth = np.array(["Smooth" in n for n, _ in systems])
+    a, b = Ragged(systems), Ragged(systems)
+    cache = str(tmp_path / "tune.json")
+    res = a.autotune(events=1, blocks=(8, 32), cache=cache, smooth=smooth)
+    assert res and all(not r["cached"] for r in res.values())
+    assert any(r["path"] == "tiles" for r in res.values())
+    for g, r in res.items():
+        info = a.group_info(g)
+        if r["choice"][0] == "launch":
+            assert list(info["launch"]) == list(r["choice"][1])
+        else:
+            assert info["block"] == r["choice"][1]
+    for rg in (a, b):
+        rg.quasistatic(1e-3, 5, active=~smooth)
+        rg.quasistatic(1e-3, 3, kicks_only=True, active=smooth)
+    for x, y in zip(a.u, b.u):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.inc, b.inc)
+    c = Ragged(systems)
+    res2 = c.autotune(cache=cache, smooth=smooth)
+    assert all(r["cached"] for r in res2.values())
+    assert [r["choice"] for r in res2.values()] == [r["choice"] for r in res.values()]
