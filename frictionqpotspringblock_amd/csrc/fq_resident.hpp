@@ -68,12 +68,34 @@ __device__ __forceinline__ double uniform_d(double x)
     return __builtin_bit_cast(double, uniform_ll(__builtin_bit_cast(long long, x)));
 }
 
+// x of the lane that the DPP control CTRL names (both 32-bit halves moved);
+// lanes of rows outside ROW_MASK receive 0.0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wavefront, in a fixed order, with DPP lane
+// moves (VALU only, no LDS round trips): pairs within each quad, the two
+// quads of each half-row, the two halves of each row (every lane then holds
+// its row's sum), rows 0 + 1 and 2 + 3 (row_bcast:15), and the two halves
+// (row_bcast:31). Lane 63 ends with the total, which every lane receives.
 __device__ __forceinline__ double wave_sum(double x)
 {
-    for (int off = 32; off > 0; off >>= 1) {
-        x += __shfl_xor(x, off, 64);
-    }
-    return x;
+    x += dpp_move<0xB1, 0xf>(x);  // quad_perm [1, 0, 3, 2]
+    x += dpp_move<0x4E, 0xf>(x);  // quad_perm [2, 3, 0, 1]
+    x += dpp_move<0x141, 0xf>(x); // row_half_mirror
+    x += dpp_move<0x140, 0xf>(x); // row_mirror
+    x += dpp_move<0x142, 0xa>(x); // row_bcast:15 into rows 1 and 3
+    x += dpp_move<0x143, 0xc>(x); // row_bcast:31 into rows 2 and 3
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 __device__ __forceinline__ long long wave_sum_i(long long x)
