@@ -871,6 +871,49 @@ def run_c0(args):
     }
 
 
+# FLOP per particle-step counted on the reference's expressions (the Verlet
+# step with two correctors and the force assembly): Cuspy + Laplace1d 40;
+# Quartic1d adds 9 (two cubes and their sum), SemiSmooth 10 (its three
+# branches' parabolas), Smooth 16 plus 20 for the sine; Laplace2d adds 2 (four
+# neighbours); a no-passing sweep (detail.h:1711-1741) 24 per particle.
+POT_FLOP = {0: 0, 1: 10, 2: 36}
+INT_FLOP = {1: 0, 2: 9, 5: 2}
+
+
+def rg_group_roofline(sets, wall):
+    """The ragged run's roofline from its own launch groups: every group's
+    time on its stream (events around its part of each operation) and its
+    particle-steps; the dominant group (the longest) against the FP64 peak,
+    and all groups together over the wall clock."""
+    groups = []
+    for rs in sets:
+        for g, (ms, ps) in enumerate(rs.group_times()):
+            if ms <= 0.0 or ps <= 0.0:
+                continue
+            info = rs.group_info(g)
+            rank, pot, inter, ext, minim = (int(x) for x in info["key"].split("-")[:5])
+            flop = 24.0 if minim == 1 else 40.0 + POT_FLOP.get(pot, 0) + INT_FLOP.get(inter, 0)
+            groups.append({"key": info["key"], "path": info["path"], "N": info["N"], "systems": len(info["systems"]),
+                           "ms": ms, "particle_steps": ps, "flop_per_particle_step": flop,
+                           "tflops": flop * ps / (ms * 1e-3) / 1e12})
+    top = max(groups, key=lambda x: x["ms"])
+    total_flop = sum(x["flop_per_particle_step"] * x["particle_steps"] for x in groups)
+    return {
+        "kernel": f"launch group {top['key']} ({top['path']}, {top['systems']} systems up to N = {top['N']}), "
+        "the ragged run's longest",
+        "bound": "valu",
+        "achieved": top["tflops"],
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": top["tflops"] / FP64_PEAK_TFLOPS,
+        "traffic": None,
+        "group_ms": top["ms"],
+        "flop_per_particle_step": top["flop_per_particle_step"],
+        "all_groups_over_wall_tflops": total_flop / wall / 1e12,
+        "groups": sorted(groups, key=lambda x: -x["ms"])[:8],
+    }
+
+
 def rg_tile_roofline(replicas=8, n=65536, steps=2048):
     """The ragged workload's dominant kernel on its own: the blocked tiled
     path (k_tile_block, K steps per launch, with its decision kernel) on
@@ -934,6 +977,8 @@ def run_ragged(args):
         events(1, args.warmup)
     first = 1 + 2 * args.warmup
     inc0 = np.array(rg.inc, dtype=float)
+    rg.reset_group_times()
+    rgn.reset_group_times()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     events(first, args.steps)
@@ -947,7 +992,8 @@ def run_ragged(args):
         work += float(np.sum(nsizes * rgn.last_steps))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    roof = rg_tile_roofline()
+    roof = rg_group_roofline([rg, rgn], wall)
+    roof["synthetic_tile_leg"] = rg_tile_roofline()
     cpu = None if args.no_cpu else cpu_leg("ragged", args)
     return {
         "metric": METRIC,
@@ -980,6 +1026,7 @@ def run_ragged(args):
 
 
 def run_flow(args):
+    blk = args.block or BLOCK
     import torch
 
     from frictionqpotspringblock_amd.batch import Batch
@@ -989,33 +1036,34 @@ def run_flow(args):
     batch = Batch(R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **kw)
     tuned = {}
     if not args.no_autotune:
-        tuned = batch.autotune(V_FRAME, steps=BLOCK)
+        tuned = batch.autotune(V_FRAME, steps=blk)
         # the profiled shape (its HBM counters are in FLOW_TRAFFIC_BYTES)
         # unless another one is clearly faster: trial times within 2% are noise
         if (4, 1024) in tuned and tuned[(4, 1024)] <= 1.02 * min(tuned.values()):
             batch.set_launch(4, 1024)
     for _ in range(args.warmup):
-        batch.flowSteps(BLOCK, V_FRAME, block=BLOCK)
+        batch.flowSteps(blk, V_FRAME, block=blk)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev_ms = batch.flowSteps(args.steps * BLOCK, V_FRAME, block=BLOCK, timed=True)
+    ev_ms = batch.flowSteps(args.steps * blk, V_FRAME, block=blk, timed=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     # every replica took every step, and its state is finite
     inc = batch.inc
-    assert np.all(inc == inc[0]) and inc[0] >= BLOCK * args.steps, inc[:4]
+    assert np.all(inc == inc[0]) and inc[0] >= blk * args.steps, inc[:4]
     u = batch.u
     assert np.all(np.isfinite(u[:: max(1, R // 16)])) and np.all(np.isfinite(batch.u_frame))
 
-    particle_steps = R * N * BLOCK * args.steps
+    particle_steps = R * N * blk * args.steps
     value = particle_steps / wall
     kernel_ms = ev_ms / args.steps  # average duration of one resident launch
-    flop = FLOP_PER_PARTICLE_STEP * R * N * BLOCK
+    flop = FLOP_PER_PARTICLE_STEP * R * N * blk
     achieved = flop / (kernel_ms * 1e-3) / 1e12
 
     plain = None if args.no_plain else plain_path()
     cpu = None if args.no_cpu else cpu_leg("flow", args)
     ppt, threads = batch.launch
+    profiled = (ppt, threads) == (4, 1024) and blk == BLOCK
     return {
         "metric": METRIC,
         "value": value,
@@ -1031,10 +1079,10 @@ def run_flow(args):
         "data": "synthetic (Weibull yield landscape from pcg32 seeds 0..1023)",
         "config": {
             "workload": f"batched flow: {R} Line1d Cuspy-Laplace replicas x N={N}, "
-            f"{BLOCK}-step resident launches, v_frame={V_FRAME}",
+            f"{blk}-step resident launches, v_frame={V_FRAME}",
             "replicas": R,
             "N": N,
-            "flow_steps": BLOCK * args.steps,
+            "flow_steps": blk * args.steps,
             "nchunk": 200,
             "launch": {"particles_per_thread": ppt, "threads": threads},
             "autotune_ms": {f"{p}x{t}": round(ms, 3) for (p, t), ms in tuned.items()},
@@ -1047,10 +1095,12 @@ def run_flow(args):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": FLOW_TRAFFIC_BYTES if (ppt, threads) == (4, 1024) else None,
+            "traffic": FLOW_TRAFFIC_BYTES if profiled else None,
+            "traffic_source": "profiles/k_flow_cuspy_laplace_flow_R1024_N4096.md (rocprofv3 --pmc FETCH_SIZE, "
+            "WRITE_SIZE of the 4 x 1024 launch of 1000 steps)" if profiled else None,
             "kernel": f"k_flow<Cuspy, Laplace1d, {ppt} x {threads}>",
             "kernel_ms": kernel_ms,
-            "hbm_gbs": FLOW_TRAFFIC_BYTES / (kernel_ms * 1e-3) / 1e9 if (ppt, threads) == (4, 1024) else None,
+            "hbm_gbs": FLOW_TRAFFIC_BYTES / (kernel_ms * 1e-3) / 1e9 if profiled else None,
         },
         "cpu_baseline": cpu,
     }
@@ -1157,6 +1207,7 @@ def main():
         "ragged: quasistatic events, default 50; quasistatic: event pairs, default 100; c0: ignored)",
     )
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--block", type=int, default=None, help="flow: steps per resident launch (default 1000)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU work per oracle build of a CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-system path")

@@ -243,6 +243,8 @@ SIGNATURES = {
     "fqsb_ragged_chunk_index_at_align": (C.c_int, [_P, _ip]),
     "fqsb_ragged_group_info": (C.c_int, [_P, C.c_uint64, _ip]),
     "fqsb_ragged_group_systems": (C.c_int, [_P, C.c_uint64, _up]),
+    "fqsb_ragged_group_time": (C.c_int, [_P, C.c_uint64, _dp, _dp]),
+    "fqsb_ragged_reset_group_time": (C.c_int, [_P]),
     "fqsb_ragged_group_shapes": (
         C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
     ),

@@ -263,25 +263,47 @@ __global__ void __launch_bounds__(ADJ_REV_THREADS) k_adj_reverse(const SysView* 
         return (q < 0 || q >= E) ? self : s[q];
     };
 
-    for (int n = steps - 1; n >= 0; --n) {
+    // the tape of step n is loaded while step n + 1 is reversed (registers
+    // qu, qv, qa, qj, quf), so its memory latency is hidden behind that work
+    double qu[SLOTS], qv[SLOTS], qa[SLOTS], quf = 0.0;
+    int32_t qj[SLOTS];
+    auto load = [&](int n) {
         const int64_t tn = (int64_t)n * Rnp + (int64_t)r * ab.np;
-        const double uf = ab.tuf[(int64_t)n * ab.R + r];
+#pragma unroll
+        for (int i = 0; i < SLOTS; ++i) {
+            qu[i] = qv[i] = qa[i] = 0.0;
+            qj[i] = 0;
+            if (live[i]) {
+                const int64_t q = tn + pp[i];
+                qu[i] = ab.tu[q];
+                qv[i] = ab.tv[q];
+                qa[i] = ab.ta[q];
+                qj[i] = ab.tj[q];
+            }
+        }
+        quf = ab.tuf[(int64_t)n * ab.R + r];
+    };
+    if (steps > 0) {
+        load(steps - 1);
+    }
+    for (int n = steps - 1; n >= 0; --n) {
+        const double uf = quf;
         double x[SLOTS], pv[SLOTS], pa[SLOTS];
         int32_t pj[SLOTS];
 #pragma unroll
         for (int i = 0; i < SLOTS; ++i) {
-            x[i] = pv[i] = pa[i] = 0.0;
-            pj[i] = 0;
+            x[i] = qu[i];
+            pv[i] = qv[i];
+            pa[i] = qa[i];
+            pj[i] = qj[i];
             if (live[i]) {
-                const int64_t q = tn + pp[i];
-                x[i] = ab.tu[q];
-                pv[i] = ab.tv[q];
-                pa[i] = ab.ta[q];
-                pj[i] = ab.tj[q];
                 su[t + T * i] = x[i];
             }
         }
         __syncthreads();
+        if (n > 0) {
+            load(n - 1);
+        }
 #pragma unroll
         for (int i = 0; i < SLOTS; ++i) {
             if (live[i]) {

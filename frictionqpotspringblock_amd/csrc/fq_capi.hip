@@ -172,6 +172,11 @@ struct Group {
     std::vector<fq::RunState> hrs;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // ragged sets: time of the group's part of every operation on its stream
+    // (events around it) and the particle-steps (no-passing: particle-sweeps)
+    // it took, summed until reset
+    hipEvent_t evr[2] = {nullptr, nullptr};
+    double busy_ms = 0.0, busy_ps = 0.0;
     int ppt = 0, maxt = 0, threads = 0;
     bool streaming = false; // systems too large for a resident workgroup
     // tiled per-step path for streaming groups (fq_tile.hpp); FQSB_NO_TILES=1
@@ -280,6 +285,11 @@ struct Group {
         dfree(tb_first);
         dfree(lr_yl);
         dfree(lr_yr);
+        for (auto& e : evr) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
         if (ev0) {
             (void)hipEventDestroy(ev0);
         }
@@ -2121,6 +2131,14 @@ static void nopassing_tiled(Group& g, const fq::RunArgs& ra, uint64_t max_iter)
     HIPC(hipStreamSynchronize(g.stream));
 }
 
+// particle-sweeps of the last no-passing run (ragged sets' work measure)
+static void count_sweeps(Group& g)
+{
+    for (int64_t r = 0; r < g.R; ++r) {
+        g.busy_ps += (double)(g.off[(size_t)r + 1] - g.off[(size_t)r]) * (double)g.hrs[(size_t)r].steps;
+    }
+}
+
 std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64_t max_iter)
 {
     for (auto& r : g.hrs) {
@@ -2136,6 +2154,7 @@ std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64
         ra.tol2 = tol * tol;
         ra.niter_tol = (int64_t)niter_tol;
         nopassing_tiled(g, ra, max_iter);
+        count_sweeps(g);
         std::vector<bool> conv((size_t)g.R);
         for (int64_t r = 0; r < g.R; ++r) {
             conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
@@ -2177,6 +2196,7 @@ std::vector<bool> nopassing_run(Group& g, double tol, uint64_t niter_tol, uint64
         done += nb;
         g.pull();
     }
+    count_sweeps(g);
     std::vector<bool> conv((size_t)g.R);
     for (int64_t r = 0; r < g.R; ++r) {
         conv[(size_t)r] = g.hrs[(size_t)r].status == fq::ST_CONVERGED;
@@ -2251,6 +2271,35 @@ struct fqsb_ragged {
     std::vector<int64_t> sizes;                    // N of system i
     std::vector<int64_t> uoff;                     // first element of system i in concatenated arrays
 
+    // the group's part of an operation between two events on its stream, and
+    // the particle-steps it took (increments of inc; no-passing sweeps are
+    // counted where they run)
+    template <class F>
+    static void timed(Group& g, F&& op)
+    {
+        for (auto& e : g.evr) {
+            if (!e) {
+                HIPC(hipEventCreate(&e));
+            }
+        }
+        std::vector<int64_t> inc0(g.hrs.size());
+        for (size_t k = 0; k < g.hrs.size(); ++k) {
+            inc0[k] = g.hrs[k].inc;
+        }
+        HIPC(hipEventRecord(g.evr[0], g.stream));
+        op();
+        HIPC(hipEventRecord(g.evr[1], g.stream));
+        HIPC(hipEventSynchronize(g.evr[1]));
+        float ms = 0.0f;
+        HIPC(hipEventElapsedTime(&ms, g.evr[0], g.evr[1]));
+        g.busy_ms += ms;
+        if (g.minim != FQSB_MIN_OVERDAMPED) {
+            for (size_t k = 0; k < g.hrs.size(); ++k) {
+                g.busy_ps += (double)(g.off[k + 1] - g.off[k]) * (double)(g.hrs[k].inc - inc0[k]);
+            }
+        }
+    }
+
     // FQSB_RAGGED_SERIAL=1: run the groups one after the other from the
     // calling thread (for tools that do not follow launches from many threads)
     template <class F>
@@ -2260,7 +2309,7 @@ struct fqsb_ragged {
             if (sr[0] == '1') {
                 for (size_t gi = 0; gi < groups.size(); ++gi) {
                     HIPC(hipSetDevice(groups[gi]->device));
-                    fn(*groups[gi], gi);
+                    timed(*groups[gi], [&] { fn(*groups[gi], gi); });
                 }
                 return;
             }
@@ -2272,7 +2321,7 @@ struct fqsb_ragged {
             th.emplace_back([&, gi] {
                 try {
                     HIPC(hipSetDevice(groups[gi]->device));
-                    fn(*groups[gi], gi);
+                    timed(*groups[gi], [&] { fn(*groups[gi], gi); });
                 }
                 catch (...) {
                     err[gi] = std::current_exception();
@@ -3295,6 +3344,26 @@ int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info)
                                g.minim, g.total};
         std::copy(v, v + 12, info);
     });
+}
+
+int fqsb_ragged_group_time(const fqsb_ragged* r, uint64_t group, double* ms, double* particle_steps)
+{
+    return guard([&] {
+        if (group >= r->groups.size()) {
+            throw Error("group index out of range");
+        }
+        *ms = r->groups[(size_t)group]->busy_ms;
+        *particle_steps = r->groups[(size_t)group]->busy_ps;
+    });
+}
+
+int fqsb_ragged_reset_group_time(fqsb_ragged* r)
+{
+    for (auto& g : r->groups) {
+        g->busy_ms = 0.0;
+        g->busy_ps = 0.0;
+    }
+    return 0;
 }
 
 int fqsb_ragged_group_systems(const fqsb_ragged* r, uint64_t group, uint64_t* idx)

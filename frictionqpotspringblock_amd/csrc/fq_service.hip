@@ -58,7 +58,27 @@ hipError_t launch_set_iref(const SysView* views, int nsys, int64_t maxN, hipStre
         return true;                                                                                           \
     }
 
+static bool launch_forces_one(int pot, int inter, int ext, const SysView* views, int nsys, int32_t flags,
+                              hipStream_t stream);
+
+// Long-range systems: the interaction forces by k_lr_fint (many workgroups
+// per system) between the updates that k_forces applies first (uniform
+// displacement, quench, thermal draws) and the other forces.
 bool launch_forces(int pot, int inter, int ext, const SysView* views, int nsys, int32_t flags, hipStream_t stream)
+{
+    if (inter != INT_LONGRANGE1D || !(flags & FF_INT)) {
+        return launch_forces_one(pot, inter, ext, views, nsys, flags, stream);
+    }
+    const int32_t first = flags & (FF_ADD_U | FF_QUENCH | FF_THERMAL);
+    if (first && !launch_forces_one(pot, inter, ext, views, nsys, first, stream)) {
+        return false;
+    }
+    hipLaunchKernelGGL(k_lr_fint, dim3(64, (unsigned)nsys), dim3(SVC_THREADS), 0, stream, views);
+    return launch_forces_one(pot, inter, ext, views, nsys, flags & ~(first | FF_INT), stream);
+}
+
+static bool launch_forces_one(int pot, int inter, int ext, const SysView* views, int nsys, int32_t flags,
+                              hipStream_t stream)
 {
     FQ_FORCES(POT_CUSPY, INT_LAPLACE1D, 0)
     FQ_FORCES(POT_CUSPY, INT_LAPLACE1D, 1)

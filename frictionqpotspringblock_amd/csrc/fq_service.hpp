@@ -109,6 +109,21 @@ __device__ void thermal_update_block(const SysView& S, RunState* rs)
     }
 }
 
+// LongRange1d interaction forces of every system (blockIdx.y) from its slips
+// u, each thread one particle's sum over the whole chain in the reference's
+// order (detail.h:849-867), the chain spread over many workgroups: the O(N^2)
+// part of k_forces for long-range systems.
+__global__ void __launch_bounds__(SVC_THREADS) k_lr_fint(const SysView* __restrict__ views)
+{
+    const SysView& S = views[blockIdx.y];
+    const int64_t N = S.N;
+    const double* __restrict__ u = S.u;
+    const double* __restrict__ pref = S.pref;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        S.f_int[p] = longrange_force(S, [&](int64_t q) { return u[q]; }, [&](int64_t d) { return pref[d]; }, p);
+    }
+}
+
 template <int POT, int INT, int EXT>
 __global__ void __launch_bounds__(SVC_THREADS) k_forces(const SysView* __restrict__ views, int32_t flags)
 {

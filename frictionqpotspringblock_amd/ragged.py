@@ -157,6 +157,19 @@ class Ragged:
             "shapes": [(int(ppt[i]), int(thr[i])) for i in range(min(cnt.value, 32))],
         }
 
+    def group_times(self):
+        """[(ms, particle_steps)] per group since the last reset: the time of
+        the group's part of every operation on its stream and the work it did."""
+        out = []
+        for g in range(self.groups):
+            ms, ps = C.c_double(), C.c_double()
+            _check(capi().fqsb_ragged_group_time(self._h, g, C.byref(ms), C.byref(ps)))
+            out.append((ms.value, ps.value))
+        return out
+
+    def reset_group_times(self):
+        _check(capi().fqsb_ragged_reset_group_time(self._h))
+
     def set_group_launch(self, g, ppt, threads):
         _check(capi().fqsb_ragged_set_group_launch(self._h, int(g), int(ppt), int(threads)))
 

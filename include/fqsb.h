@@ -420,6 +420,11 @@ enum {
  * length of the tiled path, rank, potential, interactions, external,
  * minimisation, particles. */
 int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info);
+/* Time (ms) of the group's part of every operation since the last reset,
+ * between HIP events on the group's stream, and the particle-steps (no-passing
+ * groups: particle-sweeps) it took. */
+int fqsb_ragged_group_time(const fqsb_ragged* r, uint64_t group, double* ms, double* particle_steps);
+int fqsb_ragged_reset_group_time(fqsb_ragged* r);
 /* Indices (in the order of descs) of the group's systems: info[0] values. */
 int fqsb_ragged_group_systems(const fqsb_ragged* r, uint64_t group, uint64_t* idx);
 /* Resident launch shapes the group can use (at most cap; *count: all). */
