@@ -245,6 +245,7 @@ SIGNATURES = {
     "fqsb_ragged_group_systems": (C.c_int, [_P, C.c_uint64, _up]),
     "fqsb_ragged_group_time": (C.c_int, [_P, C.c_uint64, _dp, _dp]),
     "fqsb_ragged_reset_group_time": (C.c_int, [_P]),
+    "fqsb_ragged_set_serial": (C.c_int, [_P, C.c_int]),
     "fqsb_ragged_group_shapes": (
         C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
     ),

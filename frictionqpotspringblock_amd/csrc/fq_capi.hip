@@ -191,7 +191,9 @@ struct Group {
     // blocks of tile_k steps per launch for chains (fq_tile.hpp k_tile_block;
     // FQSB_TILE_K=0: one launch per step)
     int tile_k = 0;
-    int tb_ntiles = 0;                // tiles of the blocked path (2048 particles)
+    int tb_ntiles = 0;                // tiles of the blocked path (chains: 1024 particles; lattices: rows)
+    int tb_kmax = fq::TB_KMAX_STEPS;  // longest block the tiles' halos allow (lattices: TL_K_DEFAULT)
+    size_t tb_lds = 0;                // dynamic LDS of a lattice block launch
     fq::TileDesc* tb_tiles = nullptr;
     int32_t* tb_first = nullptr;      // [R + 1]
     double* tile_part_k = nullptr;    // [tb_ntiles][tile_k][2]
@@ -504,6 +506,13 @@ struct Group {
         }
     }
 
+    // steps per block of the blocked tiled path: the requested length within
+    // what the tiles' halos allow
+    int block_steps() const
+    {
+        return std::min(tile_k, tb_kmax);
+    }
+
     bool any_running() const
     {
         for (const auto& r : hrs) {
@@ -549,15 +558,15 @@ struct Group {
             push();
         };
         auto enqueue = [&](uint64_t nb) {
-            if (blocks) { // blocks of tile_k steps (decide commits or redoes)
-                for (uint64_t s = 0; s < (nb + (uint64_t)tile_k - 1) / (uint64_t)tile_k; ++s) {
-                    if (!fq::launch_tile_block(pot, inter, views, tb_tiles, tb_ntiles, tile_part_k, tile_fail, ra,
-                                               tile_k, stream)) {
+            if (blocks) { // blocks of K steps (decide commits or redoes)
+                const int K = block_steps();
+                for (uint64_t s = 0; s < (nb + (uint64_t)K - 1) / (uint64_t)K; ++s) {
+                    if (!fq::launch_tile_block(pot, inter, views, tb_tiles, tb_ntiles, tile_part_k, tile_fail, ra, K,
+                                               tb_lds, stream)) {
                         throw Error("no blocked tile kernel for this potential / interaction");
                     }
                     HIPC(hipGetLastError());
-                    HIPC(fq::launch_tile_block_decide(views, (int)R, tb_first, tile_part_k, tile_fail, ra, tile_k,
-                                                      stream));
+                    HIPC(fq::launch_tile_block_decide(views, (int)R, tb_first, tile_part_k, tile_fail, ra, K, stream));
                 }
                 return;
             }
@@ -1854,11 +1863,46 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
             if (const char* e = std::getenv("FQSB_TILE_K")) {
                 g->tile_k = std::max(0, std::min(std::atoi(e), fq::TB_KMAX_STEPS));
             }
+            const bool lat = inter == FQSB_INT_LAPLACE2D || inter == FQSB_INT_QUARTICGRADIENT2D;
+            if (lat) { // lattices: K halo rows and columns on every side of a tile
+                int klat = fq::TL_K_DEFAULT;
+                if (const char* e = std::getenv("FQSB_LATTICE_K")) {
+                    klat = std::atoi(e);
+                }
+                g->tb_kmax = std::max(1, std::min(klat, (fq::TL_WIDTH - 1) / 2));
+            }
             if (g->tile_k > 0 && fq::tile_block_supported(g->pot, inter)) {
                 std::vector<fq::TileDesc> tb;
                 std::vector<int32_t> tbf((size_t)R + 1, 0);
+                size_t sites = 0;
+                const int64_t K = g->tb_kmax;
                 for (int64_t r = 0; r < R; ++r) {
                     tbf[(size_t)r] = (int32_t)tb.size();
+                    if (lat) {
+                        // tiles of H x W sites, (H + 2K) x (W + 2K) <= TL_SITES,
+                        // W + 2K <= TL_WIDTH, balanced over the lattice
+                        const int64_t cols = (int64_t)ds[(size_t)r].cols, rows = (int64_t)ds[(size_t)r].rows;
+                        const int64_t ntc = (cols + (fq::TL_WIDTH - 2 * K) - 1) / (fq::TL_WIDTH - 2 * K);
+                        const int64_t W = (cols + ntc - 1) / ntc;
+                        const int64_t hmax = fq::TL_SITES / (W + 2 * K) - 2 * K;
+                        const int64_t ntr = (rows + hmax - 1) / hmax;
+                        const int64_t H = (rows + ntr - 1) / ntr;
+                        for (int64_t r0 = 0; r0 < rows; r0 += H) {
+                            for (int64_t c0 = 0; c0 < cols; c0 += W) {
+                                fq::TileDesc td{};
+                                td.sys = (int32_t)r;
+                                td.r0 = (int32_t)r0;
+                                td.nr = (int32_t)std::min(H, rows - r0);
+                                td.c0 = (int32_t)c0;
+                                td.nc = (int32_t)std::min(W, cols - c0);
+                                td.p0 = r0 * cols + c0;
+                                td.p1 = td.p0 + td.nc;
+                                tb.push_back(td);
+                                sites = std::max<size_t>(sites, (size_t)((td.nr + 2 * K) * (td.nc + 2 * K)));
+                            }
+                        }
+                        continue;
+                    }
                     for (int64_t p0 = 0; p0 < Ns[(size_t)r]; p0 += fq::TB_TILE_PARTICLES) {
                         fq::TileDesc td{};
                         td.sys = (int32_t)r;
@@ -1868,6 +1912,10 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
                     }
                 }
                 tbf[(size_t)R] = (int32_t)tb.size();
+                if (sites > (size_t)fq::TL_SITES) {
+                    throw Error("internal: a lattice tile exceeds the block kernel's sites");
+                }
+                g->tb_lds = 2 * sizeof(double) * sites;
                 g->tb_ntiles = (int)tb.size();
                 g->tb_tiles = dalloc<fq::TileDesc>(tb.size());
                 g->h2d(g->tb_tiles, tb.data(), tb.size());
@@ -2270,6 +2318,7 @@ struct fqsb_ragged {
     std::vector<std::pair<size_t, int64_t>> where; // system i -> (group, index in group)
     std::vector<int64_t> sizes;                    // N of system i
     std::vector<int64_t> uoff;                     // first element of system i in concatenated arrays
+    bool serial = false;                           // groups one after the other (fqsb_ragged_set_serial)
 
     // the group's part of an operation between two events on its stream, and
     // the particle-steps it took (increments of inc; no-passing sweeps are
@@ -2300,19 +2349,19 @@ struct fqsb_ragged {
         }
     }
 
-    // FQSB_RAGGED_SERIAL=1: run the groups one after the other from the
-    // calling thread (for tools that do not follow launches from many threads)
+    // serial (or FQSB_RAGGED_SERIAL=1): run the groups one after the other
+    // from the calling thread (each group's time is then its own; also for
+    // tools that do not follow launches from many threads)
     template <class F>
     void each(F&& fn)
     {
-        if (const char* sr = std::getenv("FQSB_RAGGED_SERIAL")) {
-            if (sr[0] == '1') {
-                for (size_t gi = 0; gi < groups.size(); ++gi) {
-                    HIPC(hipSetDevice(groups[gi]->device));
-                    timed(*groups[gi], [&] { fn(*groups[gi], gi); });
-                }
-                return;
+        const char* sr = std::getenv("FQSB_RAGGED_SERIAL");
+        if (serial || (sr && sr[0] == '1')) {
+            for (size_t gi = 0; gi < groups.size(); ++gi) {
+                HIPC(hipSetDevice(groups[gi]->device));
+                timed(*groups[gi], [&] { fn(*groups[gi], gi); });
             }
+            return;
         }
         std::vector<std::exception_ptr> err(groups.size());
         std::vector<std::thread> th;
@@ -3340,9 +3389,12 @@ int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info)
         else if (g.streaming) {
             path = FQSB_PATH_STREAMING;
         }
-        const int64_t v[12] = {g.R, g.N, path, g.ppt, g.threads, g.tile_k, (int64_t)g.d.rank, g.pot, g.inter, g.ext,
-                               g.minim, g.total};
-        std::copy(v, v + 12, info);
+        const bool blocked = g.tiled && g.tile_part_k != nullptr;
+        const int64_t v[13] = {g.R,         g.N,     path,    g.ppt,   g.threads,
+                               blocked ? g.block_steps() : 0,  (int64_t)g.d.rank,
+                               g.pot,       g.inter, g.ext,   g.minim, g.total,
+                               blocked ? g.tb_kmax : 0};
+        std::copy(v, v + 13, info);
     });
 }
 
@@ -3355,6 +3407,11 @@ int fqsb_ragged_group_time(const fqsb_ragged* r, uint64_t group, double* ms, dou
         *ms = r->groups[(size_t)group]->busy_ms;
         *particle_steps = r->groups[(size_t)group]->busy_ps;
     });
+}
+
+int fqsb_ragged_set_serial(fqsb_ragged* r, int serial)
+{
+    return guard([&] { r->serial = serial != 0; });
 }
 
 int fqsb_ragged_reset_group_time(fqsb_ragged* r)

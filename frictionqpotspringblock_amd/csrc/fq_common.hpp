@@ -158,18 +158,29 @@ struct SysView {
     RunState* rs;
 };
 
-// Uniform arguments of one resident launch.
-// One tile of the tiled per-step path (fq_tile.hpp): particles [p0, p1) of
-// system `sys` (whole rows for lattices).
-// particles per tile of the blocked tiled path (k_tile_block), and the
-// longest block of steps its extended segment allows
+// particles per tile of the blocked tiled path of chains (k_tile_block), and
+// the longest block of steps its extended segment allows
 constexpr int64_t TB_TILE_PARTICLES = 1024;
 constexpr int TB_KMAX_STEPS = 128;
 
+// blocked tiled path of lattices (k_tile_block2d): a tile of rows x columns
+// plus K halo rows and columns on every side holds at most TL_SITES sites
+// (TL_PPT per thread of TL_THREADS) and is at most TL_WIDTH sites wide;
+// halos (and longest blocks) of TL_K_DEFAULT steps unless set otherwise
+constexpr int TL_THREADS = 1024;
+constexpr int TL_PPT = 4;
+constexpr int TL_SITES = TL_THREADS * TL_PPT;
+constexpr int TL_WIDTH = 64;
+constexpr int TL_K_DEFAULT = 4;
+
+// One tile of the tiled paths (fq_tile.hpp): particles [p0, p1) of system
+// `sys` (whole rows for lattices); tiles of the lattice blocks also give
+// their rows [r0, r0 + nr) and columns [c0, c0 + nc).
 struct TileDesc {
     int32_t sys;
     int32_t pad;
     int64_t p0, p1;
+    int32_t r0, nr, c0, nc;
 };
 
 // Extra device pointers of a batched long-range group (all [R, N], row r =

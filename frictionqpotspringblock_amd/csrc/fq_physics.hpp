@@ -91,11 +91,13 @@ FQ_HD int64_t wrap(int64_t i, int64_t n)
     return i < 0 ? i + n : (i >= n ? i - n : i);
 }
 
-// Lattice interactions of site (i, j) read from a row-major slip field U.
+// Lattice interactions of site (i, j) read from a row-major slip field U;
+// S supplies rows, cols (the periodic wrap) and k, k2, k4 (the system, or the
+// extended rows of a tile).
 //   Laplace2d         detail.h:545-583
 //   QuarticGradient2d detail.h:682-750
-template <int INT, class Acc>
-FQ_HD double lattice_force(const SysView& S, const Acc& U, int64_t i, int64_t j)
+template <int INT, class Sys, class Acc>
+FQ_HD double lattice_force(const Sys& S, const Acc& U, int64_t i, int64_t j)
 {
     const int64_t R = S.rows, C = S.cols;
     const int64_t im = wrap(i - 1, R), ip = wrap(i + 1, R), jm = wrap(j - 1, C), jp = wrap(j + 1, C);

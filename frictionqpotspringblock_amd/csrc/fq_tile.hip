@@ -52,11 +52,12 @@ hipError_t launch_tile_decide(const SysView* views, int nsys, const int32_t* til
 bool tile_block_supported(int pot, int inter)
 {
     (void)pot;
-    return inter == INT_NONE || inter == INT_LAPLACE1D || inter == INT_QUARTIC1D || inter == INT_QUARTICGRADIENT1D;
+    return inter == INT_NONE || inter == INT_LAPLACE1D || inter == INT_QUARTIC1D || inter == INT_QUARTICGRADIENT1D ||
+           inter == INT_LAPLACE2D || inter == INT_QUARTICGRADIENT2D;
 }
 
 bool launch_tile_block(int pot, int inter, const SysView* views, const TileDesc* tiles, int ntiles, double* part,
-                       int32_t* tfail, const RunArgs& ra, int K, hipStream_t stream)
+                       int32_t* tfail, const RunArgs& ra, int K, size_t lds, hipStream_t stream)
 {
 #define FQ_BLOCK(P, I)                                                                                         \
     if (pot == P && inter == I) {                                                                              \
@@ -64,12 +65,19 @@ bool launch_tile_block(int pot, int inter, const SysView* views, const TileDesc*
                            tiles, part, tfail, ra, K);                                                         \
         return true;                                                                                           \
     }
-#define FQ_BLOCK_CHAINS(P) FQ_BLOCK(P, INT_NONE) FQ_BLOCK(P, INT_LAPLACE1D) FQ_BLOCK(P, INT_QUARTIC1D)         \
-    FQ_BLOCK(P, INT_QUARTICGRADIENT1D)
-    FQ_BLOCK_CHAINS(POT_CUSPY)
-    FQ_BLOCK_CHAINS(POT_SEMISMOOTH)
-    FQ_BLOCK_CHAINS(POT_SMOOTH)
-#undef FQ_BLOCK_CHAINS
+#define FQ_BLOCK2D(P, I)                                                                                       \
+    if (pot == P && inter == I) {                                                                              \
+        hipLaunchKernelGGL((k_tile_block2d<P, I>), dim3((unsigned)ntiles), dim3(TL_THREADS), lds, stream,      \
+                           views, tiles, part, tfail, ra, K);                                                  \
+        return true;                                                                                           \
+    }
+#define FQ_BLOCK_ALL(P) FQ_BLOCK(P, INT_NONE) FQ_BLOCK(P, INT_LAPLACE1D) FQ_BLOCK(P, INT_QUARTIC1D)            \
+    FQ_BLOCK(P, INT_QUARTICGRADIENT1D) FQ_BLOCK2D(P, INT_LAPLACE2D) FQ_BLOCK2D(P, INT_QUARTICGRADIENT2D)
+    FQ_BLOCK_ALL(POT_CUSPY)
+    FQ_BLOCK_ALL(POT_SEMISMOOTH)
+    FQ_BLOCK_ALL(POT_SMOOTH)
+#undef FQ_BLOCK_ALL
+#undef FQ_BLOCK2D
 #undef FQ_BLOCK
     return false;
 }

@@ -548,6 +548,258 @@ __global__ void __launch_bounds__(TB_THREADS) k_tile_block(const SysView* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_tile_block2d: blocks of K steps over tiles of lattices (Laplace2d,
+// QuarticGradient2d), the 2-d form of k_tile_block. A tile owns rows
+// [r0, r0 + nr) x columns [c0, c0 + nc) and recomputes K halo rows and
+// columns on every side (periodic; a halo may repeat sites of a small
+// lattice, each copy is computed on its own): He x We = (nr + 2K) x (nc + 2K)
+// extended sites, at most TL_SITES, site i = k * T + t row-major (a wavefront
+// reads and writes runs of consecutive particles of a row). Velocity,
+// acceleration and well stay in registers; the slip of each step goes to LDS
+// (two buffers, one barrier per step), where the neighbours read it. The
+// state of step s - 1 is exact on the extended rows and columns
+// [s - 1, extent - s + 1): only those sites take step s (the others would only
+// produce values that no exact site reads). Failure codes, residual partial
+// sums, the decision and the redo are those of k_tile_block
+// (k_tile_block_decide).
+// ---------------------------------------------------------------------------
+
+// the extended sites of a tile as the lattice lattice_force wraps around
+// (opposite edges read each other: finite values, and edge sites are no
+// longer exact after the first step)
+struct TileRows {
+    int64_t rows, cols;
+    double k, k2, k4;
+};
+
+template <int POT, int INT>
+__global__ void __launch_bounds__(TL_THREADS) k_tile_block2d(const SysView* __restrict__ views,
+                                                             const TileDesc* __restrict__ tiles,
+                                                             double* __restrict__ part,
+                                                             int32_t* __restrict__ tfail, RunArgs ra, int K)
+{
+    constexpr int PPT = TL_PPT;
+    constexpr int T = TL_THREADS;
+    constexpr int nw = T / 64;
+    extern __shared__ double s_x[]; // [2][He * We]: the slips after the even and the odd steps
+    __shared__ double s_red[2][nw][2];
+    __shared__ int s_fail;
+    const TileDesc td = tiles[blockIdx.x];
+    const SysView& S = views[td.sys];
+    const RunState* const rs = S.rs;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int32_t status = uniform_i(rs->status);
+    if (status != ST_RUNNING && status != ST_BUDGET) {
+        return;
+    }
+    const int Ke = uniform_i(tb_steps(rs, K));
+    const int cb = uniform_i(rs->cur), nb = cb ^ 1;
+    // tile geometry (uniform)
+    const int R = uniform_i((int)S.rows), C = uniform_i((int)S.cols);
+    const int nr = uniform_i(td.nr), nc = uniform_i(td.nc);
+    const int He = nr + 2 * K, We = nc + 2 * K; // extended rows and columns
+    const int next = He * We;                   // extended sites (<= TL_SITES, checked by the host)
+    const int gr0 = uniform_i(((td.r0 - K) % R + R) % R); // global row and column of extended site 0
+    const int gc0 = uniform_i(((td.c0 - K) % C + C) % C);
+    const Landscape& L = S.ls;
+    const int64_t nch = L.nchunk;
+    const int32_t jmin = (int32_t)L.buffer;
+    const int32_t jmax = (int32_t)(L.nchunk - 2 - L.buffer);
+    const double* const Uc = cb ? S.u_b : S.u;
+    const double* const Vc = cb ? S.v_b : S.v;
+    const double* const Ac = cb ? S.a_b : S.a;
+    double* const Un = nb ? S.u_b : S.u;
+    double* const Vn = nb ? S.v_b : S.v;
+    double* const An = nb ? S.a_b : S.a;
+    const double dt = uniform_d(S.dt);
+    const double c2 = 0.5 * dt * dt;
+    const double hdt = 0.5 * dt;
+    const double meta = -uniform_d(S.eta);
+    const double inv_m = uniform_d(S.inv_m);
+    const double kf = uniform_d(S.k_frame);
+    const double dv = ra.v_frame * dt;
+    const bool mini = ra.mode == MODE_MINIMISE;
+    const TileRows geo{He, We, uniform_d(S.k), uniform_d(S.k2), uniform_d(S.k4)};
+
+    // per site: velocity, acceleration, well, extended row << 16 | column
+    // (He, We < 2^13); the slip lives in LDS only (the buffer of the last
+    // step taken)
+    double v[PPT], a[PPT], yl[PPT], yr[PPT];
+    int32_t jl[PPT], ec[PPT];
+    auto gidx = [&](int32_t rc) -> int64_t { // global particle of extended site rc
+        const int g = (gr0 + (rc >> 16)) % R;
+        const int c = (gc0 + (rc & 0xffff)) % C;
+        return (int64_t)g * C + c;
+    };
+    // the sites taking step s: extended row and column in [s - 1, extent - s + 1)
+    auto takes = [&](int32_t rc, int s) -> bool {
+        const int e = rc >> 16, c = rc & 0xffff;
+        return e >= s - 1 && e < He - s + 1 && c >= s - 1 && c < We - s + 1;
+    };
+    auto owns = [&](int32_t rc) -> bool {
+        const int e = rc >> 16, c = rc & 0xffff;
+        return e >= K && e < K + nr && c >= K && c < K + nc;
+    };
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int i = k * T + t;
+        v[k] = a[k] = yl[k] = yr[k] = 0.0;
+        jl[k] = 0;
+        ec[k] = -1; // no site
+        if (i < next) {
+            const int e = i / We;
+            ec[k] = (e << 16) | (i - e * We);
+            const int64_t q = gidx(ec[k]);
+            const double u0 = Uc[q];
+            v[k] = Vc[q];
+            a[k] = Ac[q];
+            // the well from jloc and the window (as in k_tile_block)
+            jl[k] = L.jloc[q];
+            yl[k] = L.y[q * nch + jl[k]];
+            yr[k] = L.y[q * nch + jl[k] + 1];
+            // both buffers start with finite slips: an edge site of the exact
+            // region reads neighbours that no longer take steps
+            s_x[i] = u0;
+            s_x[next + i] = u0;
+        }
+    }
+    if (t == 0) {
+        s_fail = TB_NO_FAIL;
+    }
+    double uf = rs->u_frame;
+    __syncthreads();
+
+    int s;
+    for (s = 1; s <= Ke; ++s) {
+        const int b = s & 1;
+        double* const xs = s_x + b * next;
+        const double* const xp = s_x + (b ^ 1) * next;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int i = k * T + t;
+            if (ec[k] >= 0 && takes(ec[k], s)) {
+                xs[i] = xp[i] + dt * v[k] + c2 * a[k];
+            }
+        }
+        __syncthreads();
+        if (mini && s > 1 && t == 0) { // residual sums of step s - 1
+            double a0 = 0.0, a1 = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                a0 += s_red[b ^ 1][w][0];
+                a1 += s_red[b ^ 1][w][1];
+            }
+            part[((int64_t)blockIdx.x * K + (s - 2)) * 2] = a0;
+            part[((int64_t)blockIdx.x * K + (s - 2)) * 2 + 1] = a1;
+        }
+        if ((uniform_i(s_fail) >> 2) < s) {
+            break; // a site of the tile failed at an earlier step
+        }
+        if (ra.flow) {
+            uf = uf + dv;
+        }
+        double sf = 0.0, sff = 0.0;
+        int fail = 0;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int i = k * T + t;
+            const int32_t eck = ec[k];
+            if (eck >= 0 && takes(eck, s)) {
+                const bool own = owns(eck);
+                const double xk = xs[i];
+                if (own && __builtin_isnan(xk)) {
+                    fail |= TILE_FLAG_NAN;
+                }
+                if (!(yl[k] <= xk && xk < yr[k]) && __builtin_isfinite(xk)) {
+                    const double* row = L.y + gidx(eck) * nch;
+                    int32_t j = jl[k];
+                    double wl = yl[k], wr = yr[k];
+                    while (xk >= wr && j < jmax) {
+                        ++j;
+                        wl = wr;
+                        wr = row[j + 1];
+                    }
+                    while (xk < wl && j > jmin) {
+                        --j;
+                        wr = wl;
+                        wl = row[j];
+                    }
+                    if (wl <= xk && xk < wr) {
+                        jl[k] = j;
+                        yl[k] = wl;
+                        yr[k] = wr;
+                    }
+                    else if (own) {
+                        fail |= TILE_FLAG_REFILL;
+                    }
+                }
+                const double fi = lattice_force<INT>(geo, [&](int64_t q) { return xs[q]; }, eck >> 16, eck & 0xffff);
+                const double fp = potential_force<POT>(S, xk, yl[k], yr[k]);
+                const double ff = kf * (uf - xk);
+                const double sum3 = ff + fp + fi;
+                const double vn = v[k], an = a[k];
+                double vv = vn + dt * an;
+                double f = sum3 + meta * vv;
+                double aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                vv = vn + hdt * (an + aa);
+                f = sum3 + meta * vv;
+                aa = f * inv_m;
+                v[k] = vv;
+                a[k] = aa;
+                if (own) {
+                    sf += f * f;
+                    sff += ff * ff;
+                }
+            }
+        }
+        if (fail) { // a refill before a NaN of the same step
+            atomicMin(&s_fail, (s << 2) | ((fail & TILE_FLAG_REFILL) ? TILE_FLAG_REFILL : TILE_FLAG_NAN));
+        }
+        if (mini) {
+            sf = wave_sum(sf);
+            sff = wave_sum(sff);
+            if (lane == 0) {
+                s_red[b][wave][0] = sf;
+                s_red[b][wave][1] = sff;
+            }
+        }
+    }
+    const int done = s - 1;
+    __syncthreads();
+    if (t == 0) {
+        if (mini && done >= 1 && done == Ke) { // sums of the last step
+            const int lb = done & 1;
+            double a0 = 0.0, a1 = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                a0 += s_red[lb][w][0];
+                a1 += s_red[lb][w][1];
+            }
+            part[((int64_t)blockIdx.x * K + (done - 1)) * 2] = a0;
+            part[((int64_t)blockIdx.x * K + (done - 1)) * 2 + 1] = a1;
+        }
+        tfail[blockIdx.x] = s_fail;
+    }
+    // the tile's own sites: state after the block into the other buffer (used
+    // only if the decision commits all Ke steps), wells in place
+    const double* const xd = s_x + (done & 1) * next;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int i = k * T + t;
+        if (ec[k] >= 0 && owns(ec[k])) {
+            const int64_t q = gidx(ec[k]);
+            Un[q] = xd[i];
+            Vn[q] = v[k];
+            An[q] = a[k];
+            L.jloc[q] = jl[k];
+            S.yl_a[q] = yl[k];
+            S.yr_a[q] = yr[k];
+        }
+    }
+}
+
 // Commit (or ask for a redo of) the block of every system: one wavefront per
 // system; lane l sums the tiles' partial norms of steps l, l + 64, ... (over
 // the tiles in order), then lane 0 runs the StopList over the steps in order.

@@ -61,6 +61,7 @@ class Ragged:
         _check(capi().fqsb_ragged_create(arr, n, C.byref(h)))
         self._h = h
         self._systems = [(name, dict(kw)) for name, kw in systems]
+        self._serial = False
         self._n = n
         self._sizes = [int(np.prod(s)) for s in self._shapes]
         self._offsets = np.concatenate([[0], np.cumsum(self._sizes)]).astype(int)
@@ -138,7 +139,7 @@ class Ragged:
     def group_info(self, g):
         """Launch group g: its systems (indices), largest size, path and
         current launch shape / block length."""
-        info = np.empty(12, dtype=np.int64)
+        info = np.empty(13, dtype=np.int64)
         _check(capi().fqsb_ragged_group_info(self._h, int(g), info.ctypes.data_as(_ip)))
         idx = np.empty(int(info[0]), dtype=np.uint64)
         _check(capi().fqsb_ragged_group_systems(self._h, int(g), idx.ctypes.data_as(_up)))
@@ -153,6 +154,7 @@ class Ragged:
             "path": self.PATHS.get(int(info[2]), str(info[2])),
             "launch": (int(info[3]), int(info[4])),
             "block": int(info[5]),
+            "block_max": int(info[12]),
             "key": "-".join(str(int(x)) for x in info[6:11]) + f"-{int(np.ceil(np.log2(max(int(info[1]), 1))))}",
             "shapes": [(int(ppt[i]), int(thr[i])) for i in range(min(cnt.value, 32))],
         }
@@ -169,6 +171,18 @@ class Ragged:
 
     def reset_group_times(self):
         _check(capi().fqsb_ragged_reset_group_time(self._h))
+
+    @property
+    def serial(self):
+        """Run the launch groups one after the other (True) instead of
+        concurrently on their streams (False, the default); results are the
+        same, and each group's time is then its own."""
+        return self._serial
+
+    @serial.setter
+    def serial(self, value):
+        _check(capi().fqsb_ragged_set_serial(self._h, int(bool(value))))
+        self._serial = bool(value)
 
     def set_group_launch(self, g, ppt, threads):
         _check(capi().fqsb_ragged_set_group_launch(self._h, int(g), int(ppt), int(threads)))
@@ -207,7 +221,15 @@ class Ragged:
             if info["path"] == "resident" and len(info["shapes"]) > 1:
                 cands = [("launch", sh) for sh in info["shapes"]]
             elif info["path"] == "tiles" and info["block"] > 0:
-                cands = [("block", k) for k in blocks]
+                # block lengths within what the tiles' halos allow (lattices:
+                # a few rows of halo, so shorter blocks as well)
+                kmax = info["block_max"]
+                ks = {min(k, kmax) for k in blocks}
+                if kmax < min(blocks):
+                    ks |= {max(1, kmax // 4), max(1, kmax // 2)}
+                cands = [("block", k) for k in sorted(ks)]
+                if len(cands) < 2:
+                    continue
             else:
                 continue
             if key in known:

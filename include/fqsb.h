@@ -416,21 +416,28 @@ enum {
     FQSB_PATH_NOPASSING_RESIDENT = 3,  /* no-passing sweeps, one workgroup per system */
     FQSB_PATH_NOPASSING_TILES = 4      /* no-passing sweeps over tiles */
 };
-/* info[12] = systems, largest N, path, particles per thread, threads, block
- * length of the tiled path, rank, potential, interactions, external,
- * minimisation, particles. */
+/* info[13] = systems, largest N, path, particles per thread, threads, block
+ * length of the tiled path (0: not blocked), rank, potential, interactions,
+ * external, minimisation, particles, longest block the tiles' halos allow
+ * (chains 128; lattices: K halo rows, 8 unless FQSB_LATTICE_K is set). */
 int fqsb_ragged_group_info(const fqsb_ragged* r, uint64_t group, int64_t* info);
 /* Time (ms) of the group's part of every operation since the last reset,
  * between HIP events on the group's stream, and the particle-steps (no-passing
  * groups: particle-sweeps) it took. */
 int fqsb_ragged_group_time(const fqsb_ragged* r, uint64_t group, double* ms, double* particle_steps);
 int fqsb_ragged_reset_group_time(fqsb_ragged* r);
+/* serial != 0: every operation runs the groups one after the other from the
+ * calling thread instead of concurrently on their streams (same results; each
+ * group's time is then its own, not shared with the groups running beside
+ * it). */
+int fqsb_ragged_set_serial(fqsb_ragged* r, int serial);
 /* Indices (in the order of descs) of the group's systems: info[0] values. */
 int fqsb_ragged_group_systems(const fqsb_ragged* r, uint64_t group, uint64_t* idx);
 /* Resident launch shapes the group can use (at most cap; *count: all). */
 int fqsb_ragged_group_shapes(const fqsb_ragged* r, uint64_t group, int* ppt, int* threads, int cap, int* count);
 /* Set a resident group's launch shape, or a tiled group's block length
- * (1 .. 128 steps per launch). Results do not depend on either. */
+ * (1 .. 128 steps per launch; longer than info[12] runs blocks of info[12]).
+ * Results do not depend on either. */
 int fqsb_ragged_set_group_launch(fqsb_ragged* r, uint64_t group, int ppt, int threads);
 int fqsb_ragged_set_group_block(fqsb_ragged* r, uint64_t group, int steps);
 

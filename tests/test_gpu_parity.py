@@ -29,8 +29,8 @@ def fq():
 def path(request, monkeypatch):
     """Run a test through the resident kernels, again with every system on
     the large-system path (FQSB_FORCE_STREAMING=1: the tiled kernels for
-    timeSteps / flowSteps / minimise -- blocks of 32 steps per launch for
-    chains --, the one-workgroup streaming kernel otherwise), with the tiled
+    timeSteps / flowSteps / minimise -- blocks of steps per launch for chains
+    and lattices --, the one-workgroup streaming kernel otherwise), with the tiled
     path one step per launch (FQSB_TILE_K=0), and with the tiles switched off
     (FQSB_NO_TILES=1: the streaming kernel for everything)."""
     monkeypatch.delenv("FQSB_FORCE_STREAMING", raising=False)
@@ -552,6 +552,51 @@ def test_tile_blocks(fq, oracle, monkeypatch, k):
     for s in (a, b):
         s.v = v0
         s.flowSteps(300, 0.2)
+    assert_same_state(a, b, "flowSteps")
+    assert np.array_equal(a.chunk.data, b.chunk.data)
+    assert a.minimise() == b.minimise() == 0
+    assert_same_state(a, b, "minimise")
+    for s in (a, b):
+        s.eventDrivenStep(1e-3, False)
+        s.eventDrivenStep(1e-3, True)
+    assert a.minimise() == b.minimise() == 0
+    assert_same_state(a, b, "kick + minimise")
+    for s in (a, b):
+        s.timeSteps(45)
+    assert_same_state(a, b, "timeSteps")
+
+
+LATTICE_BLOCKS = [
+    ("System_Cuspy_Laplace", (96, 80), dict(k_interactions=1.0), "4", None),
+    ("System_Cuspy_Laplace", (96, 80), dict(k_interactions=1.0), "1", None),
+    ("System_Cuspy_Laplace", (70, 130), dict(k_interactions=1.0), "9", None),
+    ("System_Cuspy_Laplace", (64, 72), dict(k_interactions=1.0), "8", "3"),  # blocks shorter than the halos
+    ("System_Cuspy_Laplace", (5, 7), dict(k_interactions=1.0), "4", None),  # halos repeat sites
+    ("System_Cuspy_QuarticGradient", (64, 72), dict(k2=1.0, k4=1.0), "4", None),
+]
+
+
+@pytest.mark.parametrize("cls,shape,extra,halo,k", LATTICE_BLOCKS,
+                         ids=[f"{c[0]}-{c[1][0]}x{c[1][1]}-K{c[3]}-{c[4]}" for c in LATTICE_BLOCKS])
+def test_lattice_tile_blocks(fq, oracle, monkeypatch, cls, shape, extra, halo, k):
+    """Blocks of steps over 2-d tiles of lattices (halo rows and columns of
+    `halo` sites, blocks of min(`k`, halo) steps) against the oracle:
+    flowSteps with crossings and refills, minimise, kick + minimise,
+    timeSteps."""
+    monkeypatch.setenv("FQSB_FORCE_STREAMING", "1")
+    monkeypatch.setenv("FQSB_LATTICE_K", halo)
+    if k is not None:
+        monkeypatch.setenv("FQSB_TILE_K", k)
+    n = int(np.prod(shape))
+    print(f"lattice blocks: {cls} {shape} halo {halo} block {k}", flush=True)
+    par = _base(n, shape=list(shape), **extra)
+    a = getattr(fq.Line2d, cls)(**par)
+    b = getattr(oracle.Line2d, cls)(**par)
+    scale = 1.0 if "Quartic" in cls else 10.0
+    v0 = np.random.default_rng(5).normal(0.0, scale, a.shape)
+    for s in (a, b):
+        s.v = v0
+        s.flowSteps(150, 0.2)
     assert_same_state(a, b, "flowSteps")
     assert np.array_equal(a.chunk.data, b.chunk.data)
     assert a.minimise() == b.minimise() == 0

@@ -175,12 +175,15 @@ def test_ragged_masks_and_errors(oracle):
 
 def test_ragged_autotune(oracle, tmp_path):
     """The tuner picks a launch shape per resident group and a block length
-    per tiled group by timing trial events on a copy of the set; the choice
-    changes no result (a tuned and an untuned set stay bit-identical), and a
-    cache file applies the same choices without trials."""
+    per tiled group (chains and a lattice on 2-d tiles, whose blocks are at
+    most its halo width) by timing trial events on a copy of the set; the
+    choice changes no result (a tuned set and an untuned one whose groups run
+    one after the other stay bit-identical), and a cache file applies the same
+    choices without trials."""
     from frictionqpotspringblock_amd.ragged import Ragged
 
-    systems = _systems() + [("System_Cuspy_Laplace", _par(5000, 12))]
+    systems = _systems() + [("System_Cuspy_Laplace", _par(5000, 12)),
+                            ("Line2d.System_Cuspy_Laplace", _par(48 * 64, 13, shape=[48, 64]))]
     systems = [x for x in systems if "Nopassing" not in x[0]]
     smooth = np.array(["Smooth" in n for n, _ in systems])
     a, b = Ragged(systems), Ragged(systems)
@@ -188,12 +191,19 @@ def test_ragged_autotune(oracle, tmp_path):
     res = a.autotune(events=1, blocks=(8, 32), cache=cache, smooth=smooth)
     assert res and all(not r["cached"] for r in res.values())
     assert any(r["path"] == "tiles" for r in res.values())
+    lattice = 0
     for g, r in res.items():
         info = a.group_info(g)
         if r["choice"][0] == "launch":
             assert list(info["launch"]) == list(r["choice"][1])
         else:
             assert info["block"] == r["choice"][1]
+            if info["key"].startswith("2-"):
+                lattice += 1
+                assert info["block_max"] == 4 and r["choice"][1] in (1, 2, 4), info
+                assert sorted(r["trials_ms"]) == ["block:1", "block:2", "block:4"]
+    assert lattice == 1
+    b.serial = True
     for rg in (a, b):
         rg.quasistatic(1e-3, 5, active=~smooth)
         rg.quasistatic(1e-3, 3, kicks_only=True, active=smooth)
