@@ -11,7 +11,9 @@ k_interactions, k_frame, eta) and y [R, N, nchunk]. Well indices are held at
 the values the trajectory used and are not differentiated.
 
 Cuspy + Laplace1d replicas (``Batch(..., potential="Cuspy",
-interactions="Laplace1d")``), N <= 4096, no external forcing. A batch holds
+interactions="Laplace1d")``) of any length (chains above 4096 particles run
+over tiles with halos of about ``window`` particles, so a window must be
+shorter than 2048 steps there), no external forcing. A batch holds
 the tape of its last forward pass only: the backward pass of an earlier
 ``run`` on the same batch (or one after any other call that changed the
 batch) raises instead of using another trajectory; chain runs on separate

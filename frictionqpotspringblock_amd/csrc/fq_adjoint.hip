@@ -23,12 +23,19 @@ bool launch_adj_record(int ppt, int threads, const SysView* views, const AdjBuff
                        double v_frame, hipStream_t stream)
 {
     (void)threads; // the kernel runs on exactly its variant's thread count
-    const size_t lds = 2 * sizeof(double) * (size_t)ab.N;
+    const int64_t seg = ab.rnt == 1 ? ab.N : ab.rtp + 2 * ab.rhalo; // particles of a workgroup's segment
+    const size_t lds = 2 * sizeof(double) * (size_t)seg;
     switch (ppt) {
 #define FQ_REC(P, M)                                                                                           \
     case P:                                                                                                    \
-        hipLaunchKernelGGL((k_adj_record<P, M>), dim3((unsigned)ab.R, (unsigned)windows), dim3(M), lds,        \
-                           stream, views, ab, flow, v_frame);                                                  \
+        if (ab.rnt == 1) {                                                                                     \
+            hipLaunchKernelGGL((k_adj_record<P, M, true>), dim3((unsigned)ab.R, (unsigned)windows), dim3(M),   \
+                               lds, stream, views, ab, flow, v_frame);                                         \
+        }                                                                                                      \
+        else {                                                                                                 \
+            hipLaunchKernelGGL((k_adj_record<P, M, false>), dim3((unsigned)(ab.R * ab.rnt), (unsigned)windows),\
+                               dim3(M), lds, stream, views, ab, flow, v_frame);                                \
+        }                                                                                                      \
         return true;
         FQ_ADJ_CASES(FQ_REC)
 #undef FQ_REC

@@ -39,21 +39,26 @@
 
 namespace fq {
 
-constexpr int ADJ_REV_THREADS = 512;
-constexpr int ADJ_REV_MAX_SLOTS = 8; // particles of the extended segment per thread (at most)
 
 // Record windows w0 + blockIdx.y (each of min(K, n - w K) steps) from their
-// start states, one workgroup per replica and window: every window is
-// independent given its checkpoint, so all of a group run at once. Thread t
-// holds particles t + T k (k < SLOTS); the slips of a step are exchanged
-// through an LDS array of the whole chain, double-buffered by step parity
-// (one barrier per step).
-template <int SLOTS, int MAXT>
+// start states, one workgroup per replica, tile and window (blockIdx.x =
+// replica * rnt + tile): every window is independent given its checkpoint,
+// so all of a group run at once. With one tile per replica the workgroup
+// holds the whole periodic chain; with several, a tile steps its particles
+// together with a halo of rhalo >= K particles on each side (recomputed from
+// their owners' checkpoint with the same expressions: the open ends' error
+// moves inwards by one particle per step and never reaches the tile's own
+// particles within a window) and writes the tape of its own particles only.
+// Thread t holds local particles t + T k (k < SLOTS) of the segment; the
+// slips of a step are exchanged through LDS, double-buffered by step parity
+// (one barrier per step). RING: one tile per replica (local = global index,
+// every particle its own: compile-time offsets of the tape entries).
+template <int SLOTS, int MAXT, bool RING>
 __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__ views, AdjBuffers ab, int flow,
                                                      double v_frame)
 {
-    extern __shared__ double sx[]; // [2][N]
-    const int r = blockIdx.x;
+    extern __shared__ double sx[]; // [2][E]
+    const int r = blockIdx.x / ab.rnt, tile = blockIdx.x % ab.rnt;
     const int64_t w = ab.w0 + blockIdx.y;
     const int steps = (int)(ab.n - w * ab.K < ab.K ? ab.n - w * ab.K : ab.K);
     // this window's start state and tape
@@ -72,6 +77,11 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     const int t = threadIdx.x;
     const int N = (int)ab.N;
     const int64_t off = (int64_t)r * N;
+    constexpr bool ring = RING;
+    const int p0 = tile * (int)ab.rtp;
+    const int p1 = p0 + (int)ab.rtp < N ? p0 + (int)ab.rtp : N;
+    const int H = ring ? 0 : (int)ab.rhalo;
+    const int E = ring ? N : (p1 - p0) + 2 * H; // particles of the segment
     const Landscape& L = S.ls;
     const double* const Y = L.y;
     const int64_t nch = L.nchunk;
@@ -80,13 +90,21 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     const int64_t Rnp = ab.R * ab.np;
 
     double u[SLOTS], v[SLOTS], a[SLOTS], yl[SLOTS], yr[SLOTS];
-    int32_t jl[SLOTS];
+    int32_t jl[SLOTS], pp[SLOTS];
+    bool own[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
         u[k] = v[k] = a[k] = yl[k] = yr[k] = 0.0;
         jl[k] = 0;
-        const int p = t + T * k;
-        if (p < N) {
+        const int e = t + T * k;
+        int p = e;
+        if constexpr (!RING) {
+            p = p0 - H + e;
+            p = p < 0 ? p + N : (p >= N ? p - N : p);
+        }
+        pp[k] = p;
+        own[k] = e < E && (RING || (e >= H && e < H + (p1 - p0)));
+        if (e < E) {
             u[k] = ab.wu[off + p];
             v[k] = ab.wv[off + p];
             a[k] = ab.wa[off + p];
@@ -101,30 +119,30 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
     int bad = 0;
 
     for (int n = 0; n < steps; ++n) {
-        double* const xb = sx + (n & 1) * N;
+        double* const xb = sx + (n & 1) * E;
         if (flow) {
             uf = uf + dv;
         }
         const int64_t tn = (int64_t)n * Rnp + (int64_t)r * ab.np;
-        if (t == 0) {
+        if (t == 0 && tile == 0) {
             ab.tuf[(int64_t)n * ab.R + r] = uf;
         }
-        double* const tv = ab.tv + tn + t;
-        double* const ta = ab.ta + tn + t;
-        double* const tu = ab.tu + tn + t;
-        int32_t* const tj = ab.tj + tn + t;
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
-            const int p = t + T * k;
-            if (p < N) {
-                const int q = T * k; // compile-time offset from this thread's entries
-                tv[q] = v[k];
-                ta[q] = a[k];
+            const int e = t + T * k;
+            if (e < E) {
+                const int64_t q = tn + (RING ? t + T * k : pp[k]);
+                if (own[k]) {
+                    ab.tv[q] = v[k];
+                    ab.ta[q] = a[k];
+                }
                 u[k] = u[k] + dt * v[k] + c2 * a[k];
-                tu[q] = u[k];
                 const double x = u[k];
+                if (own[k]) {
+                    ab.tu[q] = x;
+                }
                 if (!(yl[k] <= x && x < yr[k])) {
-                    const double* row = Y + (int64_t)p * nch;
+                    const double* row = Y + (int64_t)pp[k] * nch;
                     int32_t j = jl[k];
                     double lo = yl[k], hi = yr[k];
                     while (x >= hi && j < jmax) {
@@ -137,22 +155,33 @@ __global__ void __launch_bounds__(MAXT) k_adj_record(const SysView* __restrict__
                         hi = lo;
                         lo = row[j];
                     }
-                    bad |= !(lo <= x && x < hi);
+                    bad |= own[k] && !(lo <= x && x < hi);
                     jl[k] = j;
                     yl[k] = lo;
                     yr[k] = hi;
                 }
-                tj[q] = jl[k];
-                xb[p] = x;
+                if (own[k]) {
+                    ab.tj[q] = jl[k];
+                }
+                xb[e] = x;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
-            const int p = t + T * k;
-            if (p < N) {
-                const double um = xb[p == 0 ? N - 1 : p - 1];
-                const double up = xb[p == N - 1 ? 0 : p + 1];
+            const int e = t + T * k;
+            if (e < E) {
+                // neighbours: periodic on the ring; at the open ends of a
+                // tile's segment, any finite value (the particle's own slip)
+                double um, up;
+                if (ring) {
+                    um = xb[e == 0 ? N - 1 : e - 1];
+                    up = xb[e == N - 1 ? 0 : e + 1];
+                }
+                else {
+                    um = e == 0 ? u[k] : xb[e - 1];
+                    up = e == E - 1 ? u[k] : xb[e + 1];
+                }
                 const double fi = chain_force<INT_LAPLACE1D>(S, um, u[k], up);
                 const double fp = potential_force<POT_CUSPY>(S, u[k], yl[k], yr[k]);
                 const double ff = kf * (uf - u[k]);

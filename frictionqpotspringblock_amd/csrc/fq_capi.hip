@@ -102,6 +102,9 @@ void dfree(T*& p)
 constexpr uint64_t MIN_BLOCK = 1u << 18;
 // adjoint: device memory for the tape of the windows recorded by one launch
 constexpr size_t ADJ_TAPE_BYTES = size_t(16) << 30;
+// adjoint record: chains up to this size run whole on one workgroup (4 x 1024
+// threads); longer ones over tiles with halos
+constexpr int64_t ADJ_RECORD_RING = 4096;
 // In ragged sets, systems above this size run on the tiled path (many
 // workgroups per system): with a few hundred systems the resident path
 // leaves a large system to one workgroup, the tiled path spreads it over the
@@ -760,8 +763,8 @@ struct Group {
             throw Error("the adjoint needs replicas of one shape with the Cuspy potential, Laplace1d "
                         "interactions and no external forcing");
         }
-        if (N > 4096 || N < 3) {
-            throw Error("the adjoint supports chains of 3 to 4096 particles");
+        if (N < 3) {
+            throw Error("the adjoint needs chains of at least 3 particles");
         }
     }
 
@@ -771,6 +774,14 @@ struct Group {
         adjoint_check();
         if (K < 1 || n < 1) {
             throw Error("the adjoint needs n >= 1 steps and windows of K >= 1 steps");
+        }
+        // chains beyond one workgroup run both adjoint kernels over tiles with
+        // halos of about K particles on each side
+        const int64_t rev_cap = (int64_t)fq::ADJ_REV_THREADS * fq::ADJ_REV_MAX_SLOTS;
+        if ((N > ADJ_RECORD_RING && (int64_t)K >= ADJ_RECORD_RING / 2) ||
+            (N > rev_cap && 2 * ((int64_t)K + 1) >= rev_cap)) {
+            throw Error("the adjoint's tiles do not fit: windows of " + std::to_string(K) +
+                        " steps are too long for a chain of " + std::to_string(N) + " particles");
         }
         adj_valid = false;
         const int64_t nw = (int64_t)((n + K - 1) / K);
@@ -824,14 +835,28 @@ struct Group {
         }
         adjoint_check();
         const size_t RN = (size_t)total;
+        // the record kernel's segments: the whole periodic chain up to
+        // ADJ_RECORD_RING particles, else tiles with halos of K particles
+        // whose segments (own + 2 K) fit 4 x 1024 threads
+        int64_t rnt = 1, rtp = N, rseg = N;
+        if (N > ADJ_RECORD_RING) {
+            const int64_t cap = ADJ_RECORD_RING - 2 * adj_K;
+            if (cap < 1) {
+                throw Error("the adjoint's record tiles do not fit: window too long for the chain");
+            }
+            rnt = (N + cap - 1) / cap;
+            rtp = (N + rnt - 1) / rnt;
+            rnt = (N + rtp - 1) / rtp;
+            rseg = rtp + 2 * adj_K;
+        }
         // launch shape of the record kernel: fewest particles per thread that
-        // fits (4 x 1024 at N = 4096 holds its registers without spills; 8 x
+        // fits (4 x 1024 at 4096 holds its registers without spills; 8 x
         // 512 does not)
         int vp[8], vm[8];
         const int nv = fq::adj_variants(vp, vm, 8);
         int ppt = 0, threads = 0;
         for (int i = 0; i < nv; ++i) {
-            const int64_t need = (N + vp[i] - 1) / vp[i];
+            const int64_t need = (rseg + vp[i] - 1) / vp[i];
             const int thr = (int)(((need + 63) / 64) * 64);
             if (thr <= vm[i]) {
                 ppt = vp[i];
@@ -857,6 +882,14 @@ struct Group {
         else {
             while (R * nt < 256 && (N + 2 * nt - 1) / (2 * nt) >= 4 * halo) {
                 nt *= 2;
+            }
+        }
+        // at least as many tiles as make a tile and its halos fit the kernel
+        const int64_t rev_cap = (int64_t)fq::ADJ_REV_THREADS * fq::ADJ_REV_MAX_SLOTS;
+        if (N > rev_cap) {
+            nt = std::max<int64_t>(nt, 2);
+            while (nt < N && (N + nt - 1) / nt + 2 * halo > rev_cap) {
+                ++nt;
             }
         }
         int64_t tp = (N + nt - 1) / nt;
@@ -934,6 +967,9 @@ struct Group {
             ab.K = adj_K;
             ab.n = adj_n;
             ab.w0 = w0;
+            ab.rnt = (int32_t)rnt;
+            ab.rtp = rtp;
+            ab.rhalo = rnt == 1 ? 0 : adj_K;
             HIPC(hipEventRecord(eva[0], stream));
             if (!fq::launch_adj_record(ppt, threads, views, ab, (int)(w1 - w0 + 1), adj_flow, adj_vf, stream)) {
                 throw Error("adjoint record kernel not available");

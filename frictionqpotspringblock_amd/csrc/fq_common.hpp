@@ -171,7 +171,7 @@ constexpr int TL_THREADS = 1024;
 constexpr int TL_PPT = 4;
 constexpr int TL_SITES = TL_THREADS * TL_PPT;
 constexpr int TL_WIDTH = 64;
-constexpr int TL_K_DEFAULT = 4;
+constexpr int TL_K_DEFAULT = 8;
 
 // One tile of the tiled paths (fq_tile.hpp): particles [p0, p1) of system
 // `sys` (whole rows for lattices); tiles of the lattice blocks also give
@@ -195,6 +195,11 @@ struct LrBuffers {
     double c;   // sum of one row of the circulant matrix
     int nsplit; // number of partial products
 };
+
+// adjoint reverse pass (k_adj_reverse): threads per workgroup and the most
+// particles of a tile's extended segment per thread
+constexpr int ADJ_REV_THREADS = 512;
+constexpr int ADJ_REV_MAX_SLOTS = 8;
 
 // Buffers of the adjoint of a batch (fq_adjoint.hpp); [R*N] arrays are
 // replica after replica. The tape holds, per step, R blocks of np = N entries
@@ -234,6 +239,11 @@ struct AdjBuffers {
     int32_t nt;
     int32_t pad1;
     int64_t tp, halo;
+    // k_adj_record: tiles per replica (1: the whole periodic chain), particles
+    // per tile, halo width
+    int32_t rnt;
+    int32_t pad2;
+    int64_t rtp, rhalo;
 };
 
 struct RunArgs {

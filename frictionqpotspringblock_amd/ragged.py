@@ -195,12 +195,15 @@ class Ragged:
         length (tiled groups) by timing a few events of the examples'
         quasistatic protocol per candidate, on a copy of this set built from
         the same descriptions (so this set's own state is left as it is);
-        results do not depend on the choice. `smooth` marks the systems that
-        take kicks only (their maxUniformDisplacement raises). With `cache` (a
-        JSON file), choices found there for a group's key (class and size
-        class) are applied without trials and new ones are added. Groups of
-        fewer than `min_particles` particles keep their default. Returns
-        {group: {"key", "path", "choice", "trials_ms"}}."""
+        results do not depend on the choice. Candidates run successive events
+        (of different avalanche sizes), so they are compared by the group's
+        time per particle-step (events on its stream, the other groups idle).
+        `smooth` marks the systems that take kicks only (their
+        maxUniformDisplacement raises). With `cache` (a JSON file), choices
+        found there for a group's key (class and size class) are applied
+        without trials and new ones are added. Groups of fewer than
+        `min_particles` particles keep their default. Returns {group: {"key",
+        "path", "choice", "trials_ms", "trials_ns_per_particle_step"}}."""
         import json
         import os
         import time
@@ -225,7 +228,7 @@ class Ragged:
                 # a few rows of halo, so shorter blocks as well)
                 kmax = info["block_max"]
                 ks = {min(k, kmax) for k in blocks}
-                if kmax < min(blocks):
+                if kmax <= min(blocks):
                     ks |= {max(1, kmax // 4), max(1, kmax // 2)}
                 cands = [("block", k) for k in sorted(ks)]
                 if len(cands) < 2:
@@ -242,22 +245,27 @@ class Ragged:
                 trial.quasistatic(eps, 1)  # event 0 of every system
             act = np.zeros(self._n, dtype=bool)
             act[info["systems"]] = True
-            times = {}
+            times, rate = {}, {}
             for kind, val in cands:
                 trial._apply(g, kind, val)
+                trial.reset_group_times()
                 t0 = time.perf_counter()
                 if np.any(act & ~smooth):
                     trial.quasistatic(eps, 2 * events, first_event=first, active=act & ~smooth)
                 if np.any(act & smooth):
                     trial.quasistatic(eps, events, first_event=(first + 1) // 2, kicks_only=True, active=act & smooth)
-                times[f"{kind}:{val}"] = (time.perf_counter() - t0) * 1e3
+                name = f"{kind}:{val}"
+                times[name] = (time.perf_counter() - t0) * 1e3
+                ms, ps = trial.group_times()[g]
+                rate[name] = ms * 1e6 / ps if ps > 0 else times[name] * 1e6
                 first += 2 * events
-            best = min(times, key=times.get)
+            best = min(rate, key=rate.get)
             kind, val = best.split(":", 1)
             val = [int(x) for x in val.strip("()").split(",")] if kind == "launch" else int(val)
             self._apply(g, kind, val)
             known[key] = {"choice": [kind, val]}
-            out[g] = {"key": key, "path": info["path"], "choice": [kind, val], "trials_ms": times, "cached": False}
+            out[g] = {"key": key, "path": info["path"], "choice": [kind, val], "trials_ms": times,
+                      "trials_ns_per_particle_step": rate, "cached": False}
         if cache:
             with open(cache, "w") as f:
                 json.dump(known, f, indent=1, sort_keys=True)

@@ -61,10 +61,13 @@ def _setup(b, rng, vscale):
 
 
 @pytest.mark.parametrize("flow", [False, True])
-@pytest.mark.parametrize("N", [200, 1500, 2100, 3500])
+@pytest.mark.parametrize("N", [200, 1500, 2100, 3500, 6000, 20000])
 def test_dot_product_with_crossings(flow, N):
-    """N = 200, 1500, 2100, 3500: 1, 2, 4 and 8 particles per thread, each
-    with a partly filled last wavefront (the tape's padded layout)."""
+    """N = 200, 1500, 2100, 3500: the whole chain on one workgroup, 1, 2, 4
+    and 4 particles per thread, each with a partly filled last wavefront;
+    N = 6000 and 20000: the record kernel over 2 and 6 tiles with halos of
+    K particles, the reverse kernel over tiles that a single workgroup could
+    not hold (and at N = 20000 a forward pass on the blocked tiled path)."""
     import torch
 
     from frictionqpotspringblock_amd import adjoint as A
@@ -163,9 +166,10 @@ def test_adjoint_errors():
     b = _batch(2, 64)
     with pytest.raises(RuntimeError, match="no adjoint forward run"):
         _check(capi().fqsb_batch_adjoint_backward_device(b._h, *([None] * 8)))
+    # chains over tiles: the halos of a window must leave room for the tile
     b = _batch(1, 5000)
-    with pytest.raises(RuntimeError, match="3 to 4096"):
-        _check(capi().fqsb_batch_adjoint_forward(b._h, 10, 5, 0, 0.0))
+    with pytest.raises(RuntimeError, match="tiles do not fit"):
+        _check(capi().fqsb_batch_adjoint_forward(b._h, 4000, 2048, 0, 0.0))
     # a trajectory that needs a landscape refill is refused
     b = _batch(1, 64, nchunk=40, k_frame=0.5)
     with pytest.raises(RuntimeError, match="refill"):

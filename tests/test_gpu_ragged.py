@@ -200,8 +200,8 @@ def test_ragged_autotune(oracle, tmp_path):
             assert info["block"] == r["choice"][1]
             if info["key"].startswith("2-"):
                 lattice += 1
-                assert info["block_max"] == 4 and r["choice"][1] in (1, 2, 4), info
-                assert sorted(r["trials_ms"]) == ["block:1", "block:2", "block:4"]
+                assert info["block_max"] == 8 and r["choice"][1] in (2, 4, 8), info
+                assert sorted(r["trials_ms"]) == ["block:2", "block:4", "block:8"]
     assert lattice == 1
     b.serial = True
     for rg in (a, b):
