@@ -964,7 +964,10 @@ def run_ragged(args):
         # on a copy of the set (this set's events do not depend on it)
         t0 = time.perf_counter()
         res = rg.autotune(RG_EPS, events=1, smooth=smooth, min_particles=RG_TUNE_MIN_PARTICLES)
-        tuned = {r["key"]: {"path": r["path"], "choice": r["choice"]} for r in res.values()}
+        tuned = {r["key"]: {"path": r["path"], "choice": r["choice"],
+                            "ns_per_particle_step": {k: round(v, 4) for k, v in
+                                                     (r.get("trials_ns_per_particle_step") or {}).items()}}
+                 for r in res.values()}
         tuned["seconds"] = time.perf_counter() - t0
     nps = ragged_nopassing(systems)
     rgn = Ragged(nps)

@@ -32,6 +32,11 @@ except ImportError as e:  # pragma: no cover - exercised only without a build
         f"({e}); run `python -m frictionqpotspringblock_amd.build_ext` first"
     ) from e
 
+from . import _views  # noqa: E402
+
+# u, v, a: writable like the reference's views (item assignments reach the system)
+_views.install(_fqsb.System)
+
 Line1d = _fqsb.Line1d
 Line2d = _fqsb.Line2d
 Particles = _fqsb.Particles

@@ -75,7 +75,8 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     // the current step, the other the slips prepared for the next step (and,
     // before they are committed, the state of the previous one). The step
     // loop is unrolled by two so that the arrays swap roles without copies.
-    // Registers i >= cnt are padding: zero state (reset after every step) in
+    // Registers i >= cnt are padding: zero state (reset after every step; with
+    // FULL, threads without particles skip the step's arithmetic instead) in
     // an unbounded well, so they never walk, never request a refill, never
     // set the NaN flag and never publish; the hot loops run over all PPT
     // registers without per-particle branches and only the stores are
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
         s_refill[0] = 0;
         s_refill[1] = 0;
     }
-    int64_t inc = rs->inc;
+    int64_t inc = uniform_ll(rs->inc); // scalar: its increment is a scalar add
     double uf = rs->u_frame;
     uint64_t rng = rs->rng_state;
     const uint64_t rng_inc = rs->rng_inc;
@@ -141,11 +142,11 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     auto publish = [&](const double(&x)[PPT], int b) {
         if (cnt > 0) {
             h_first[b * T + t] = x[0];
-        }
 #pragma unroll
-        for (int i = 0; i < PPT; ++i) {
-            if (i == last) {
-                h_last[b * T + t] = x[i];
+            for (int i = 0; i < PPT; ++i) {
+                if (i == last) {
+                    h_last[b * T + t] = x[i];
+                }
             }
         }
     };
@@ -168,7 +169,10 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             int refill = 0;
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                const double z = y[i];
+                // opaque copy: the comparisons of this rare path stay in it
+                // instead of being evaluated for every step
+                double z = y[i];
+                __asm__ volatile("" : "+v"(z));
                 if (!(yl[i] <= z && z < yr[i]) && __builtin_isfinite(z)) {
                     refill |= !reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], z, jmin, jmax);
                 }
@@ -297,6 +301,11 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             }
         }
 
+        // FULL: a thread without particles skips the arithmetic (its
+        // registers stay at rest: zero state in an unbounded well)
+        if (FULL && cnt == 0) {
+            return true;
+        }
         // the reference's three velocity / acceleration passes (detail.h:1552-1565)
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
@@ -317,10 +326,10 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             v[i] = vv;
             a[i] = aa;
         }
-        if (cnt < PPT) { // padding registers stay at rest (FULL: all of them)
+        if (!FULL && cnt < PPT) { // padding registers stay at rest
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
-                if (FULL || i >= cnt) {
+                if (i >= cnt) {
                     v[i] = 0.0;
                     a[i] = 0.0;
                 }

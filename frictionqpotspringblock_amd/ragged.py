@@ -190,7 +190,8 @@ class Ragged:
     def set_group_block(self, g, steps):
         _check(capi().fqsb_ragged_set_group_block(self._h, int(g), int(steps)))
 
-    def autotune(self, eps=1e-3, events=2, blocks=(16, 32, 64, 128), cache=None, smooth=None, min_particles=0):
+    def autotune(self, eps=1e-3, events=2, blocks=(16, 32, 64, 128), cache=None, smooth=None, min_particles=0,
+                 margin=0.1):
         """Choose each launch group's launch shape (resident groups) or block
         length (tiled groups) by timing a few events of the examples'
         quasistatic protocol per candidate, on a copy of this set built from
@@ -202,7 +203,11 @@ class Ragged:
         maxUniformDisplacement raises). With `cache` (a JSON file), choices
         found there for a group's key (class and size class) are applied
         without trials and new ones are added. Groups of fewer than
-        `min_particles` particles keep their default. Returns {group: {"key",
+        `min_particles` particles keep their default. A trial runs the group
+        alone, while a set's groups normally share the GPU (and its few
+        hardware queues), where more and shorter launches cost more than the
+        trial shows: a candidate replaces the group's default only if it is
+        faster by more than `margin` (a fraction). Returns {group: {"key",
         "path", "choice", "trials_ms", "trials_ns_per_particle_step"}}."""
         import json
         import os
@@ -245,6 +250,9 @@ class Ragged:
                 trial.quasistatic(eps, 1)  # event 0 of every system
             act = np.zeros(self._n, dtype=bool)
             act[info["systems"]] = True
+            default = ("launch", tuple(info["launch"])) if info["path"] == "resident" else ("block", info["block"])
+            if default not in cands:
+                cands.append(default)
             times, rate = {}, {}
             for kind, val in cands:
                 trial._apply(g, kind, val)
@@ -260,6 +268,9 @@ class Ragged:
                 rate[name] = ms * 1e6 / ps if ps > 0 else times[name] * 1e6
                 first += 2 * events
             best = min(rate, key=rate.get)
+            base = f"{default[0]}:{default[1]}"
+            if rate[best] > (1.0 - margin) * rate[base]:
+                best = base
             kind, val = best.split(":", 1)
             val = [int(x) for x in val.strip("()").split(",")] if kind == "launch" else int(val)
             self._apply(g, kind, val)

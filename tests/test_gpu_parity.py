@@ -461,6 +461,33 @@ def test_flow_full_threads(fq, oracle, N):
         assert b.u_frame[r] == o.u_frame
 
 
+def test_state_views_write_back(fq, oracle):
+    """u, v, a behave like the reference's views for writes: item and slice
+    assignments (also augmented, also through a kept reference) reach the
+    system; the oracle gets the same values by whole-array assignment."""
+    par = _base(300, k_interactions=1.0)
+    a = fq.Line1d.System_Cuspy_Laplace(**par)
+    b = oracle.Line1d.System_Cuspy_Laplace(**par)
+    a.u[3] = a.u[3] + 0.7
+    a.u[10:20] += 0.05
+    v = a.v
+    v[7] = 0.3
+    v[100:110] -= 0.2
+    a.a[5] = 0.01
+    u, vb, ab = b.u, b.v, b.a
+    u[3] += 0.7
+    u[10:20] += 0.05
+    vb[7] = 0.3
+    vb[100:110] -= 0.2
+    ab[5] = 0.01
+    b.u, b.v, b.a = u, vb, ab
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v) and np.array_equal(a.a, b.a)
+    for s in (a, b):
+        s.timeSteps(50)
+    assert_same_state(a, b, "after writes through views")
+    assert type(a.u + 1.0) is np.ndarray
+
+
 def test_errors(fq):
     s = fq.Line1d.System_Cuspy_Laplace(**P.delta_par(5))
     s.u = np.full(5, np.nan)
