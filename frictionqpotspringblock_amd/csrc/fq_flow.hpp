@@ -42,6 +42,9 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
 {
     static_assert(is_chain<INT>(), "k_flow handles chains");
     const SysView& S = views[blockIdx.x];
+    // the system's parameters as a local copy: registers for the whole launch
+    // instead of a reload of the ones the forces use after every barrier
+    const SysView P = S;
     RunState* const rs = S.rs;
     if (rs->status != ST_RUNNING && rs->status != ST_BUDGET) {
         return;
@@ -75,12 +78,12 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     // the current step, the other the slips prepared for the next step (and,
     // before they are committed, the state of the previous one). The step
     // loop is unrolled by two so that the arrays swap roles without copies.
-    // Registers i >= cnt are padding: zero state (reset after every step; with
-    // FULL, threads without particles skip the step's arithmetic instead) in
+    // Registers i >= cnt are padding: zero state (reset after every step) in
     // an unbounded well, so they never walk, never request a refill, never
     // set the NaN flag and never publish; the hot loops run over all PPT
     // registers without per-particle branches and only the stores are
-    // guarded.
+    // guarded. The reset runs only in wavefronts that hold padding (a scalar
+    // branch: a full wavefront skips it).
     double sa[PPT], sb[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], ym[PPT], fth[PPT];
     int32_t jl[PPT];
 #pragma unroll
@@ -120,6 +123,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
     }
     int64_t inc = uniform_ll(rs->inc); // scalar: its increment is a scalar add
     double uf = rs->u_frame;
+    const bool pad_wave = uniform_i(__any(cnt < PPT)) != 0;
     uint64_t rng = rs->rng_state;
     const uint64_t rng_inc = rs->rng_inc;
     int64_t min_next = rs->min_next;
@@ -199,7 +203,7 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
                     up = (q == last) ? right_of(b) : x[q + 1 < PPT ? q + 1 : q];
                 }
             }
-            return chain_force<INT>(S, um, x[i], up);
+            return chain_force<INT>(P, um, x[i], up);
         }
     };
 
@@ -301,16 +305,11 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             }
         }
 
-        // FULL: a thread without particles skips the arithmetic (its
-        // registers stay at rest: zero state in an unbounded well)
-        if (FULL && cnt == 0) {
-            return true;
-        }
         // the reference's three velocity / acceleration passes (detail.h:1552-1565)
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const double fi = interaction(i, b, x);
-            const double fp = potential_force_mid<POT>(S, x[i], yl[i], yr[i], ym[i]);
+            const double fp = potential_force_mid<POT>(P, x[i], yl[i], yr[i], ym[i]);
             const double ff = kf * (uf - x[i]);
             const double sum3 = ff + fp + fi;
             const double vn = v[i], an = a[i];
@@ -326,12 +325,17 @@ __global__ void __launch_bounds__(MAXT) k_flow(const SysView* __restrict__ views
             v[i] = vv;
             a[i] = aa;
         }
-        if (!FULL && cnt < PPT) { // padding registers stay at rest
+        if (pad_wave) { // padding registers stay at rest (FULL: all of them)
+            // (the empty asm keeps this branch apart from the per-lane one
+            // below, so that it stays a scalar branch)
+            __asm__ volatile("");
+            if (cnt < PPT) {
 #pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                if (i >= cnt) {
-                    v[i] = 0.0;
-                    a[i] = 0.0;
+                for (int i = 0; i < PPT; ++i) {
+                    if (FULL || i >= cnt) {
+                        v[i] = 0.0;
+                        a[i] = 0.0;
+                    }
                 }
             }
         }

@@ -327,6 +327,7 @@ __global__ void __launch_bounds__(TB_THREADS) k_tile_block(const SysView* __rest
     __shared__ int s_fail;
     const TileDesc td = tiles[blockIdx.x];
     const SysView& S = views[td.sys];
+    const SysView P = S; // parameters in registers (no reload after each barrier)
     const RunState* const rs = S.rs;
     const int t = threadIdx.x, T = blockDim.x, lane = t & 63, wave = t >> 6, nw = T >> 6;
     const int32_t status = uniform_i(rs->status);
@@ -481,9 +482,9 @@ __global__ void __launch_bounds__(TB_THREADS) k_tile_block(const SysView* __rest
                             up = (q == cnt - 1) ? right : x[q + 1 < PPT ? q + 1 : q];
                         }
                     }
-                    fi = chain_force<INT>(S, um, xk, up);
+                    fi = chain_force<INT>(P, um, xk, up);
                 }
-                const double fp = potential_force<POT>(S, xk, yl[k], yr[k]);
+                const double fp = potential_force<POT>(P, xk, yl[k], yr[k]);
                 const double ff = kf * (uf - xk);
                 const double sum3 = ff + fp + fi;
                 const double vn = v[k], an = a[k];
@@ -587,6 +588,7 @@ __global__ void __launch_bounds__(TL_THREADS) k_tile_block2d(const SysView* __re
     __shared__ int s_fail;
     const TileDesc td = tiles[blockIdx.x];
     const SysView& S = views[td.sys];
+    const SysView P = S; // parameters in registers (no reload after each barrier)
     const RunState* const rs = S.rs;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int32_t status = uniform_i(rs->status);
@@ -734,7 +736,7 @@ __global__ void __launch_bounds__(TL_THREADS) k_tile_block2d(const SysView* __re
                     }
                 }
                 const double fi = lattice_force<INT>(geo, [&](int64_t q) { return xs[q]; }, eck >> 16, eck & 0xffff);
-                const double fp = potential_force<POT>(S, xk, yl[k], yr[k]);
+                const double fp = potential_force<POT>(P, xk, yl[k], yr[k]);
                 const double ff = kf * (uf - xk);
                 const double sum3 = ff + fp + fi;
                 const double vn = v[k], an = a[k];
