@@ -74,7 +74,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 # each scaled by the ratio of the true bytes of a 1 GiB device copy to what
 # the same counter reads for it (reads x 2.000, writes x 1.000);
 # profiles/k_flow_cuspy_laplace_flow_R1024_N4096.md
-FLOW_TRAFFIC_BYTES = (502_477 * 2.000 + 1_030_076 * 1.000) * 1024
+FLOW_TRAFFIC_BYTES = (520_371 * 2.000 + 869_879 * 1.000) * 1024
 
 LR_R = 256
 LR_N = 8192
@@ -82,6 +82,10 @@ LR_BLOCK = 100
 
 RG_EPS = 1e-3
 RG_TUNE_MIN_PARTICLES = 262144  # groups tuned by the bench: at least this many particles
+# a trial choice replaces a group's default only if faster by this fraction:
+# the trials time each group alone, and choices that won by 10-15% there
+# (shorter blocks) made the concurrent run 2-5% slower
+RG_TUNE_MARGIN = 0.25
 QS_R = 1024
 QS_N = 1024
 QS_EPS = 1e-3
@@ -963,7 +967,8 @@ def run_ragged(args):
         # launch shape / block length of the larger groups, from trial events
         # on a copy of the set (this set's events do not depend on it)
         t0 = time.perf_counter()
-        res = rg.autotune(RG_EPS, events=1, smooth=smooth, min_particles=RG_TUNE_MIN_PARTICLES)
+        res = rg.autotune(RG_EPS, events=1, smooth=smooth, min_particles=RG_TUNE_MIN_PARTICLES,
+                          margin=RG_TUNE_MARGIN)
         tuned = {r["key"]: {"path": r["path"], "choice": r["choice"],
                             "ns_per_particle_step": {k: round(v, 4) for k, v in
                                                      (r.get("trials_ns_per_particle_step") or {}).items()}}

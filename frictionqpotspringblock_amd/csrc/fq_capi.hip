@@ -3369,13 +3369,17 @@ int fqsb_ragged_create(const fqsb_desc* descs, uint64_t n, fqsb_ragged** out)
             classes[Key(d.rank, d.potential, d.interactions, d.external, d.minimisation, lg)].push_back(i);
         }
         rg->where.resize((size_t)n);
+        int64_t resident_max = RAGGED_RESIDENT_MAX;
+        if (const char* e = std::getenv("FQSB_RAGGED_RESIDENT_MAX")) {
+            resident_max = std::max<int64_t>(0, std::atoll(e));
+        }
         for (auto& kv : classes) {
             std::vector<fqsb_desc> ds;
             for (uint64_t i : kv.second) {
                 rg->where[(size_t)i] = std::make_pair(rg->groups.size(), (int64_t)ds.size());
                 ds.push_back(descs[i]);
             }
-            rg->groups.emplace_back(build_group_multi(ds, RAGGED_RESIDENT_MAX));
+            rg->groups.emplace_back(build_group_multi(ds, resident_max));
         }
         *out = rg.release();
     });
