@@ -1,7 +1,8 @@
 """Build the native parts of frictionqpotspringblock_amd in-tree.
 
 * ``libfqsb.so``: the HIP kernels for gfx950 and the C ABI (include/fqsb.h),
-  compiled with hipcc, one translation unit per process.
+  compiled with hipcc, one translation unit per process; the resident-kernel
+  instantiation units are generated first (csrc/gen_instances.py).
 * ``_fqsb.<ext>.so``: the pybind11 module over the C++ host classes
   (include/fqsb.hpp), compiled with g++ and linked against libfqsb.so.
 
@@ -48,12 +49,18 @@ HIP_SOURCES = [
     "fq_longrange.hip",
     "fq_adjoint.hip",
     "fq_tile.hip",
-    "fq_res_cuspy_laplace.hip",
-    "fq_res_cuspy_chain.hip",
-    "fq_res_cuspy_field.hip",
-    "fq_res_semismooth.hip",
-    "fq_res_smooth.hip",
 ]
+
+
+def _generate_units():
+    """The resident-kernel instantiation units (fq_res_*.hip, fq_res_units.hpp)
+    are generated from the table in csrc/gen_instances.py."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("_fq_gen_instances", os.path.join(CSRC, "gen_instances.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.generate(CSRC)
 
 
 def _run(cmd):
@@ -71,8 +78,9 @@ def _hip_object(src):
 
 def build_lib(jobs: int = 8) -> str:
     os.makedirs(BUILD, exist_ok=True)
+    sources = HIP_SOURCES + _generate_units()
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_hip_object, HIP_SOURCES))
+        objs = list(ex.map(_hip_object, sources))
     out = os.path.join(HERE, "libfqsb.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
