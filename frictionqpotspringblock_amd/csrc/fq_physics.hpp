@@ -38,6 +38,54 @@ FQ_HD double potential_force(const SysView& S, double u, double yl, double yr)
     }
 }
 
+// The same forces with the constants of the current well computed once per
+// well (when it changes) instead of every step: Cuspy its mid-point (c0);
+// SemiSmooth the edges u_l (c0), u_r (c1) of the parabolic part (two
+// divisions); Smooth the prefactor (-mu dy) / pi of the sine (c0). Each
+// constant is the subexpression of potential_force above, so the force is
+// bit-identical.
+template <int POT, class Sys>
+FQ_HD void well_consts(const Sys& S, double yl, double yr, double& c0, double& c1)
+{
+    if constexpr (POT == POT_CUSPY) {
+        c0 = 0.5 * (yl + yr);
+        c1 = 0.0;
+    }
+    else if constexpr (POT == POT_SEMISMOOTH) {
+        const double mu = S.mu, kappa = S.kappa;
+        double xi = 0.5 * (yl + yr);
+        c0 = (mu * xi + kappa * yl) / (mu + kappa);
+        c1 = (mu * xi + kappa * yr) / (mu + kappa);
+    }
+    else {
+        double dy = 0.5 * (yr - yl);
+        c0 = -S.mu * dy / FQ_PI;
+        c1 = 0.0;
+    }
+}
+
+template <int POT, class Sys>
+FQ_HD double potential_force_wc(const Sys& S, double u, double yl, double yr, double c0, double c1)
+{
+    if constexpr (POT == POT_CUSPY) {
+        return (c0 - u) * S.mu;
+    }
+    else if constexpr (POT == POT_SEMISMOOTH) {
+        if (u < c0) {
+            return S.kappa * (u - yl);
+        }
+        if (u <= c1) {
+            return S.mu * (0.5 * (yl + yr) - u);
+        }
+        return S.kappa * (u - yr);
+    }
+    else {
+        double umin = 0.5 * (yl + yr);
+        double dy = 0.5 * (yr - yl);
+        return c0 * fq_sin(FQ_PI * (u - umin) / dy);
+    }
+}
+
 // maxUniformDisplacement of one particle in its current well (yl <= u < yr):
 //   Cuspy      detail.h:177-187  distance to the well's edge in `direction`
 //   SemiSmooth detail.h:283-330  distance to the edge of the parabolic part;
