@@ -1539,14 +1539,16 @@ struct Group {
             ra.max_steps = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), (int64_t)MIN_BLOCK));
         }
         const size_t lds = fq::resident_lds(inter, N, threads);
-        bool full = true;
+        bool unit_mass = true, full = true;
         for (const auto& S : hviews) {
+            unit_mass &= S.inv_m == 1.0;
             full &= S.N % ppt == 0;
         }
         push();
         HIPC(hipEventRecord(ev0, stream));
         for (bool first_launch = true;; first_launch = false) {
-            if (!fq::launch_quasi(pot, inter, ppt, maxt, threads, lds, views, (int)R, ra, qa, stream, full)) {
+            if (!fq::launch_quasi(pot, inter, ppt, maxt, threads, lds, views, (int)R, ra, qa, stream, unit_mass,
+                                  full)) {
                 if (first_launch) {
                     return false;
                 }

@@ -39,6 +39,7 @@ SYSTEMS = [
     ("Line1d", "System_Cuspy_Laplace", 1000, dict(k_interactions=1.0)),
     ("Line1d", "System_Cuspy_Laplace", 2000, dict(k_interactions=1.0)),
     ("Line1d", "System_Cuspy_Laplace", 3001, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 2000, dict(k_interactions=1.0, m=2.0)),
     ("Line1d", "System_SemiSmooth_Laplace", 700, dict(k_interactions=1.0, kappa=1.0)),
     ("Line1d", "System_Cuspy_Quartic", 900, dict(a1=1.0, a2=5.0)),
     ("Line1d", "System_Cuspy_QuarticGradient", 900, dict(k2=1.0, k4=2.0)),
@@ -50,7 +51,9 @@ SYSTEMS = [
 ]
 
 
-@pytest.mark.parametrize("sub,cls,n,extra", SYSTEMS, ids=[f"{s[1]}-{s[0]}-{s[2]}" for s in SYSTEMS])
+@pytest.mark.parametrize(
+    "sub,cls,n,extra", SYSTEMS, ids=[f"{s[1]}-{s[0]}-{s[2]}" + ("-m2" if "m" in s[3] else "") for s in SYSTEMS]
+)
 def test_driver_matches_protocol(fq, oracle, sub, cls, n, extra):
     """41 events on the device against the oracle's loop; nchunk 60 makes the
     driver refill landscapes both during minimisations and after kicks."""
@@ -189,6 +192,30 @@ def test_batch_replicas(fq, oracle):
         ref = P.quasistatic(o, 25)
         assert_same_events({k: v[r] for k, v in q.items()}, *ref, what=f"replica {r}")
         assert np.array_equal(batch.u[r], o.u)
+        assert np.array_equal(batch.index_at_align[r], o.chunk.index_at_align)
+
+
+@pytest.mark.parametrize("shape,m", [((8, 128), 1.0), ((4, 256), 1.0), ((16, 64), 1.0), ((8, 128), 2.0)])
+def test_batch_launch_shapes(fq, oracle, shape, m):
+    """Replicas of 1024 particles (every thread holds its full share) under
+    the launch shapes the quasistatic workload tries, with unit and other
+    mass; each replica equals a single oracle system with its seed."""
+    from frictionqpotspringblock_amd.batch import Batch
+
+    R, N = 2, 1024
+    par = _base(N, k_interactions=1.0, m=m)
+    for k in ("shape", "seed"):
+        par.pop(k)
+    batch = Batch(R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", shape=[N], seed=3, **par)
+    batch.set_launch(*shape)
+    assert batch.launch == shape
+    q = batch.quasistatic(1e-3, 21)
+    for r in range(R):
+        o = oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, m=m, seed=3 + r))
+        ref = P.quasistatic(o, 21)
+        assert_same_events({k: v[r] for k, v in q.items()}, *ref, what=f"replica {r}")
+        assert np.array_equal(batch.u[r], o.u)
+        assert np.array_equal(batch.v[r], o.v)
         assert np.array_equal(batch.index_at_align[r], o.chunk.index_at_align)
 
 
