@@ -712,16 +712,26 @@ def run_quasistatic(args):
     best = None
     if not args.no_autotune:
         # launch shapes tried on a copy of the workload (so that the timed
-        # events are the same whatever is tried): event 0, then one event pair
-        # per shape
+        # events are the same whatever is tried): event 0 and warm-up pairs (a
+        # fifth of the timed pairs, at most 20: later events have larger
+        # avalanches, with more well crossings per step, and the shapes rank
+        # differently there), then two event pairs per shape, compared by
+        # kernel time per particle-step
         trial = Batch(QS_R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **qs_params(0))
         trial.quasistatic(QS_EPS, 1)
         event = 1
+        warm = min(20, args.steps // 5)
+        if warm:
+            trial.set_launch(*QS_SHAPES[-1])
+            trial.quasistatic(QS_EPS, 2 * warm, first_event=event)
+            event += 2 * warm
         for ppt, threads in QS_SHAPES:
             trial.set_launch(ppt, threads)
-            trial.quasistatic(QS_EPS, 2, first_event=event)
-            event += 2
-            tune[f"{ppt}x{threads}"] = trial.quasistatic_stats[0]
+            i0 = trial.inc.copy()
+            trial.quasistatic(QS_EPS, 4, first_event=event)
+            event += 4
+            work = float(QS_N * (trial.inc - i0).sum())
+            tune[f"{ppt}x{threads}"] = trial.quasistatic_stats[0] * 1e6 / work
         best = min(tune, key=tune.get)
         del trial
     batch = Batch(QS_R, seed_stride=1, potential="Cuspy", interactions="Laplace1d", **qs_params(0))
@@ -769,12 +779,12 @@ def run_quasistatic(args):
             "time_steps_per_replica_max": int(steps.max()),
             "avalanche_size_mean": float(np.abs(q["S"][:, 1::2]).mean()),
             "launch": f"{ppt}x{threads}",
-            "launch_tuning_ms": tune,
+            "launch_tuning_ns_per_particle_step": tune,
             "driver_launches": launches,
             "plain_one_system_python_loop": plain,
         },
         "roofline": {
-            "kernel": f"k_quasi<Cuspy, Laplace1d, {ppt} x {threads}>",
+            "kernel": f"k_quasi_chain<Cuspy, Laplace1d, {ppt} x {threads}>",
             "bound": "valu",
             "achieved": achieved,
             "peak": 78.6,

@@ -15,7 +15,11 @@
 //  * padding registers (i >= cnt) at rest in an unbounded well: no
 //    per-particle branches in the step, the sums masked.
 // The event protocol (records, elastic and kick displacements, quench), the
-// in-launch refills and the step budget are those of k_quasi.
+// in-launch refills and the step budget are those of k_quasi. Between
+// minimisations the protocol works on the state in memory: the entry whose
+// minimisation converged writes the state back, the events run on it, and a
+// new entry reloads the registers, so that neither the step loop's registers
+// nor the protocol's arguments are live across each other's code.
 #pragma once
 
 #include "fq_flow.hpp"
@@ -40,6 +44,7 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     __shared__ int s_refill[2];
     __shared__ long long s_qs_i[RES_MAX_WAVES];
     __shared__ double s_qs_d[RES_MAX_WAVES];
+    __shared__ QsArgs s_qa; // the protocol's arguments, read where the events use them
     extern __shared__ double s_dyn[];
 
     const int t = threadIdx.x, T = blockDim.x, lane = t & 63, wave = t >> 6, nw = T >> 6;
@@ -64,6 +69,18 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     const double inv_m = S.inv_m;
     const double kf = S.k_frame;
     const bool pad_wave = uniform_i(__any(cnt < PPT)) != 0;
+    if (t == 0) {
+        s_qa = qa;
+    }
+    // The system's view as re-read by the paths outside the step loop
+    // (entries, events, write-back): an opaque pointer, so that the array
+    // pointers are loaded where these paths use them instead of being kept
+    // in scalar registers through the step loop.
+    auto V = [&]() -> const SysView& {
+        const SysView* q = views + blockIdx.x;
+        __asm__ volatile("" : "+s"(q));
+        return *q;
+    };
 
     int64_t inc = uniform_ll(rs->inc);
     double uf = rs->u_frame;
@@ -80,11 +97,31 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     uint32_t rmask = 0;      // this thread's particles that need a refill (bit i)
     int32_t status = ST_RUNNING;
     int limit = 0;
+    int64_t maxit = 0; // steps the entry's minimisation may take before max_iter
 
     // sa / sb: the state's slips and the slips prepared for the next step, in
     // turn; registers i >= cnt are padding
-    double sa[PPT], sb[PPT], v[PPT], a[PPT], yl[PPT], yr[PPT], ym[PPT];
+    double sa[PPT], sb[PPT], v[PPT], a[PPT], ym[PPT];
     int32_t jl[PPT];
+    // the bounds (y_l, y_r) of each register's well and, for up to 4 particles
+    // per thread (NEXT; 8 would spill), the right bound of the next well to
+    // the right (yield position j + 2; +inf past the window's end), read when
+    // the well changes: its load is not waited for until a later crossing
+    // uses it, so that a move of one well to the right (most moves of an
+    // avalanche under forward driving) reads no memory
+    constexpr bool NEXT = PPT <= 4;
+    double yl_r[PPT], yr_r[PPT], yn_r[NEXT ? PPT : 1];
+    auto next_right = [&](const double* row, int32_t j) -> double {
+        return j + 2 < nch ? row[j + 2] : __builtin_inf();
+    };
+    auto bounds = [&](int i, double& lo, double& hi) {
+        lo = yl_r[i];
+        hi = yr_r[i];
+    };
+    auto set_bounds = [&](int i, double lo, double hi) {
+        yl_r[i] = lo;
+        yr_r[i] = hi;
+    };
 
     auto publish = [&](const double(&x)[PPT], int b) {
         if (cnt > 0) {
@@ -107,17 +144,27 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             y[i] = x[i] + dt * v[i] + c2 * a[i];
-            out |= !(yl[i] <= y[i] && y[i] < yr[i]);
+            double lo, hi;
+            bounds(i, lo, hi);
+            out |= !(lo <= y[i] && y[i] < hi);
         }
         pend = out;
         uint32_t refill = 0;
         if (out) { // rare: some slip leaves its well
+            // opaque copy of the first index: the window addresses of this
+            // rare path are computed in it, not kept live through the loop
+            int64_t bq = base;
+            __asm__ volatile("" : "+v"(bq));
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 double z = y[i];
                 __asm__ volatile("" : "+v"(z));
-                if (!(yl[i] <= z && z < yr[i]) && __builtin_isfinite(z)) {
-                    if (!reachable(Y + (base + i) * nch, jl[i], yl[i], yr[i], z, jmin, jmax)) {
+                double lo, hi;
+                bounds(i, lo, hi);
+                if (!(lo <= z && z < hi) && __builtin_isfinite(z)) {
+                    // one well to the right: reachable (the walk's first move)
+                    const bool near = NEXT && z >= hi && z < yn_r[NEXT ? i : 0] && jl[i] < jmax;
+                    if (!near && !reachable(Y + (bq + i) * nch, jl[i], lo, hi, z, jmin, jmax)) {
                         refill |= 1u << i;
                     }
                 }
@@ -200,14 +247,15 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     // advanceUniformly(du, input_is_frame = false): true if some particle left
     // its window (rmask; the state is then refilled at u)
     auto qs_advance = [&](double du) __attribute__((always_inline)) -> bool {
+        const SysView& W = V();
         uint32_t need = 0;
 #pragma unroll 1
         for (int i = 0; i < cnt; ++i) {
             const int64_t p = base + i;
-            const double z = S.u[p] + du;
-            S.u[p] = z;
+            const double z = W.u[p] + du;
+            W.u[p] = z;
             const double* row = Y + p * nch;
-            int32_t j = L.jloc[p];
+            int32_t j = W.ls.jloc[p];
             double lo = row[j], hi = row[j + 1];
             if (!(lo <= z && z < hi) && __builtin_isfinite(z)) {
                 while (z >= hi && j < jmax) {
@@ -221,7 +269,7 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
                     lo = row[j];
                 }
                 if (lo <= z && z < hi) {
-                    L.jloc[p] = j;
+                    W.ls.jloc[p] = j;
                 }
                 else {
                     need |= 1u << i;
@@ -238,11 +286,12 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 
     // sum over the thread's particles of (well index - i_ref)
     auto well_sum = [&]() -> long long {
+        const SysView& W = V();
         long long sd = 0;
 #pragma unroll 1
         for (int i = 0; i < cnt; ++i) {
             const int64_t p = base + i;
-            sd += (int32_t)(L.start[p] + L.jloc[p] - S.i_ref[p]);
+            sd += (int32_t)(W.ls.start[p] + W.ls.jloc[p] - W.i_ref[p]);
         }
         return sd;
     };
@@ -252,35 +301,36 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     // minimisation starts (ST_RUNNING, counters reset), the run is complete
     // (ST_DONE) or a displacement needs a refill (ST_REFILL).
     auto qs_next = [&]() -> int32_t {
+        const SysView& W = V();
         for (;;) {
             if (phase == QS_RECORD) {
                 double sfr = 0.0;
 #pragma unroll 1
                 for (int i = 0; i < cnt; ++i) {
-                    sfr += kf * (uf - S.u[base + i]);
+                    sfr += kf * (uf - W.u[base + i]);
                 }
                 long long tot;
                 double tfr;
                 qs_sum(well_sum(), sfr, tot, tfr);
                 if (t == 0) {
-                    const int64_t o = (int64_t)blockIdx.x * qa.nevents + ev;
-                    if (qa.out_S) {
-                        qa.out_S[o] = tot - isum;
+                    const int64_t o = (int64_t)blockIdx.x * s_qa.nevents + ev;
+                    if (s_qa.out_S) {
+                        s_qa.out_S[o] = tot - isum;
                     }
-                    if (qa.out_uf) {
-                        qa.out_uf[o] = uf;
+                    if (s_qa.out_uf) {
+                        s_qa.out_uf[o] = uf;
                     }
-                    if (qa.out_ff) {
-                        qa.out_ff[o] = tfr / (double)N;
+                    if (s_qa.out_ff) {
+                        s_qa.out_ff[o] = tfr / (double)N;
                     }
-                    if (qa.out_inc) {
-                        qa.out_inc[o] = inc;
+                    if (s_qa.out_inc) {
+                        s_qa.out_inc[o] = inc;
                     }
                 }
                 ++ev;
                 phase = QS_BEGIN;
             }
-            if (ev >= qa.nevents) {
+            if (ev >= s_qa.nevents) {
                 return ST_DONE;
             }
             {
@@ -289,23 +339,23 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
                 qs_sum(well_sum(), 0.0, tot, unused);
                 isum = tot;
             }
-            const int64_t k = qa.first + ev;
+            const int64_t k = s_qa.first + ev;
             bool move = false;
             double du = 0.0;
             if (k == 0) { // initial quench: u_frame = 0, then minimise
                 uf = 0.0;
                 phase = QS_MINIMISE;
             }
-            else if ((k & 1) && !qa.kicks) { // elastic: eventDrivenStep(eps, false)
+            else if ((k & 1) && !s_qa.kicks) { // elastic: eventDrivenStep(eps, false)
                 if constexpr (POT != POT_SMOOTH) {
                     double m = __builtin_inf();
                     int outside = 0;
 #pragma unroll 1
                     for (int i = 0; i < cnt; ++i) {
                         const int64_t p = base + i;
-                        const int32_t j = L.jloc[p];
-                        const double z = max_uniform_particle<POT>(P, S.u[p], Y[p * nch + j], Y[p * nch + j + 1],
-                                                                   qa.direction, outside);
+                        const int32_t j = W.ls.jloc[p];
+                        const double z = max_uniform_particle<POT>(P, W.u[p], Y[p * nch + j], Y[p * nch + j + 1],
+                                                                   s_qa.direction, outside);
                         m = z < m ? z : m;
                     }
                     long long out;
@@ -313,15 +363,15 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
                     if (out) {
                         m = 0.0;
                     }
-                    if (!(m < 0.5 * qa.eps)) {
-                        du = qa.direction > 0 ? m - 0.5 * qa.eps : 0.5 * qa.eps - m;
+                    if (!(m < 0.5 * s_qa.eps)) {
+                        du = s_qa.direction > 0 ? m - 0.5 * s_qa.eps : 0.5 * s_qa.eps - m;
                         move = true;
                     }
                 }
                 phase = QS_RECORD;
             }
             else { // kick: eventDrivenStep(eps, true), then minimise
-                du = qa.direction > 0 ? qa.eps : -qa.eps;
+                du = s_qa.direction > 0 ? s_qa.eps : -s_qa.eps;
                 move = true;
                 phase = QS_MINIMISE;
             }
@@ -375,7 +425,7 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
                 status = ST_CONVERGED;
                 return false;
             }
-            if (ev_done0 + (it - 1) >= qa.max_iter) {
+            if (it - 1 >= maxit) {
                 status = ST_MAXITER;
                 return false;
             }
@@ -393,29 +443,44 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         // they were prepared; a non-finite slip keeps its well)
         if (pend) { // rare
             int nanf = 0;
+            int64_t bq = base; // (opaque, as in prepare)
+            __asm__ volatile("" : "+v"(bq));
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const double z = x[i];
                 nanf |= __builtin_isnan(z);
-                if (!(yl[i] <= z && z < yr[i])) {
-                    const double* row = Y + (base + i) * nch;
-                    int32_t j = jl[i];
-                    double lo = yl[i], hi = yr[i];
-                    while (z >= hi && j < jmax) {
-                        ++j;
-                        lo = hi;
-                        hi = row[j + 1];
-                    }
-                    while (z < lo && j > jmin) {
-                        --j;
-                        hi = lo;
-                        lo = row[j];
-                    }
-                    if (lo <= z && z < hi) {
+                double lo, hi;
+                bounds(i, lo, hi);
+                if (!(lo <= z && z < hi)) {
+                    const double* row = Y + (bq + i) * nch;
+                    if (NEXT && z >= hi && z < yn_r[NEXT ? i : 0] && jl[i] < jmax) { // one well to the right
+                        const int32_t j = jl[i] + 1;
+                        const double hn = yn_r[NEXT ? i : 0];
                         jl[i] = j;
-                        yl[i] = lo;
-                        yr[i] = hi;
-                        ym[i] = 0.5 * (lo + hi);
+                        set_bounds(i, hi, hn);
+                        ym[i] = 0.5 * (hi + hn);
+                        yn_r[NEXT ? i : 0] = next_right(row, j);
+                    }
+                    else {
+                        int32_t j = jl[i];
+                        while (z >= hi && j < jmax) {
+                            ++j;
+                            lo = hi;
+                            hi = row[j + 1];
+                        }
+                        while (z < lo && j > jmin) {
+                            --j;
+                            hi = lo;
+                            lo = row[j];
+                        }
+                        if (lo <= z && z < hi) {
+                            jl[i] = j;
+                            set_bounds(i, lo, hi);
+                            ym[i] = 0.5 * (lo + hi);
+                            if constexpr (NEXT) {
+                                yn_r[i] = next_right(row, j);
+                            }
+                        }
                     }
                 }
             }
@@ -430,7 +495,9 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const double fi = interaction(i, b, x);
-            const double fp = potential_force_mid<POT>(P, x[i], yl[i], yr[i], ym[i]);
+            double lo, hi;
+            bounds(i, lo, hi);
+            const double fp = potential_force_mid<POT>(P, x[i], lo, hi, ym[i]);
             const double ff = kf * (uf - x[i]);
             const double sum3 = ff + fp + fi;
             const double vn = v[i], an = a[i];
@@ -475,6 +542,7 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     // write back the state x after `done` steps of this entry, with the forces
     // of that configuration unless a NaN or a refill stopped the entry
     auto finish = [&](double(&x)[PPT], int done) {
+        const SysView& W = V();
         if (status == ST_RUNNING) {
             status = ST_BUDGET;
         }
@@ -488,20 +556,22 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         for (int i = 0; i < PPT; ++i) {
             if (own(i)) {
                 const int64_t p = base + i;
-                S.u[p] = x[i];
-                S.v[p] = v[i];
-                S.a[p] = a[i];
-                L.jloc[p] = jl[i];
+                W.u[p] = x[i];
+                W.v[p] = v[i];
+                W.a[p] = a[i];
+                W.ls.jloc[p] = jl[i];
                 if (forces) {
                     const double fi = interaction(i, fb, x);
-                    const double fp = potential_force_mid<POT>(P, x[i], yl[i], yr[i], ym[i]);
+                    double lo, hi;
+                    bounds(i, lo, hi);
+                    const double fp = potential_force_mid<POT>(P, x[i], lo, hi, ym[i]);
                     const double ff = kf * (uf - x[i]);
                     const double fd = meta * v[i];
-                    S.f_int[p] = fi;
-                    S.f_pot[p] = fp;
-                    S.f_frame[p] = ff;
-                    S.f_damp[p] = fd;
-                    S.f[p] = ff + fp + fi + fd;
+                    W.f_int[p] = fi;
+                    W.f_pot[p] = fp;
+                    W.f_frame[p] = ff;
+                    W.f_damp[p] = fd;
+                    W.f[p] = ff + fp + fi + fd;
                 }
             }
         }
@@ -517,21 +587,27 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
     for (;;) {
         const int64_t budget = ra.max_steps - launch_done;
         limit = uniform_i((int)(budget > 0 ? budget : 0));
+        const SysView& W = V();
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             sa[i] = sb[i] = v[i] = a[i] = ym[i] = 0.0;
-            yl[i] = -__builtin_inf();
-            yr[i] = __builtin_inf();
+            set_bounds(i, -__builtin_inf(), __builtin_inf());
+            if constexpr (NEXT) {
+                yn_r[i] = __builtin_inf();
+            }
             jl[i] = 0;
             if (own(i)) {
                 const int64_t p = base + i;
-                sa[i] = S.u[p];
-                v[i] = S.v[p];
-                a[i] = S.a[p];
-                jl[i] = L.jloc[p];
-                yl[i] = Y[p * nch + jl[i]];
-                yr[i] = Y[p * nch + jl[i] + 1];
-                ym[i] = 0.5 * (yl[i] + yr[i]);
+                sa[i] = W.u[p];
+                v[i] = W.v[p];
+                a[i] = W.a[p];
+                jl[i] = W.ls.jloc[p];
+                const double lo = Y[p * nch + jl[i]], hi = Y[p * nch + jl[i] + 1];
+                set_bounds(i, lo, hi);
+                ym[i] = 0.5 * (lo + hi);
+                if constexpr (NEXT) {
+                    yn_r[i] = next_right(Y + p * nch, jl[i]);
+                }
             }
         }
         if (t == 0) {
@@ -545,6 +621,7 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         publish(sa, 0);
         int it = 1;
         const bool minimising = !complete && phase == QS_MINIMISE;
+        maxit = s_qa.max_iter - ev_done0;
         if (minimising) {
             prepare(1, sa, sb);
             for (;; it += 2) { // steps in pairs, the register arrays in turn; the state ends in sa
@@ -576,10 +653,10 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
             for (int i = 0; i < PPT; ++i) {
                 if (own(i)) {
                     const int64_t p = base + i;
-                    S.u[p] = sa[i];
-                    S.v[p] = v[i];
-                    S.a[p] = a[i];
-                    L.jloc[p] = jl[i];
+                    W.u[p] = sa[i];
+                    W.v[p] = v[i];
+                    W.a[p] = a[i];
+                    W.ls.jloc[p] = jl[i];
                 }
             }
             status = qs_next();

@@ -195,12 +195,20 @@ def test_batch_replicas(fq, oracle):
         assert np.array_equal(batch.index_at_align[r], o.chunk.index_at_align)
 
 
-@pytest.mark.parametrize("shape,m", [((8, 128), 1.0), ((4, 256), 1.0), ((16, 64), 1.0), ((8, 128), 2.0)])
-def test_batch_launch_shapes(fq, oracle, shape, m):
+@pytest.mark.parametrize(
+    "shape,m,block",
+    [((8, 128), 1.0, None), ((4, 256), 1.0, None), ((16, 64), 1.0, None), ((8, 128), 2.0, None), ((4, 256), 1.0, "500")],
+)
+def test_batch_launch_shapes(fq, oracle, monkeypatch, shape, m, block):
     """Replicas of 1024 particles (every thread holds its full share) under
     the launch shapes the quasistatic workload tries, with unit and other
-    mass; each replica equals a single oracle system with its seed."""
+    mass, and in launches of 500 steps (the cached bound of the next well
+    read again at every relaunch); each replica equals a single oracle system
+    with its seed."""
     from frictionqpotspringblock_amd.batch import Batch
+
+    if block is not None:
+        monkeypatch.setenv("FQSB_QS_BLOCK", block)
 
     R, N = 2, 1024
     par = _base(N, k_interactions=1.0, m=m)
@@ -210,6 +218,8 @@ def test_batch_launch_shapes(fq, oracle, shape, m):
     batch.set_launch(*shape)
     assert batch.launch == shape
     q = batch.quasistatic(1e-3, 21)
+    if block is not None:
+        assert batch.quasistatic_stats[1] > 1  # relaunched
     for r in range(R):
         o = oracle.Line1d.System_Cuspy_Laplace(**_base(N, k_interactions=1.0, m=m, seed=3 + r))
         ref = P.quasistatic(o, 21)
