@@ -221,6 +221,7 @@ struct Group {
     double *u2 = nullptr, *v2 = nullptr, *a2 = nullptr, *yl = nullptr, *yr = nullptr, *yl2 = nullptr, *yr2 = nullptr;
     int32_t* jl2 = nullptr;
     size_t lds_limit = 65536;
+    int cus = 256; // compute units of the device (4 SIMDs each)
     // LongRange1d batches in matrix form (fq_longrange.hpp)
     int lr_method = FQSB_LR_EXACT;
     int64_t lr_m = 0;
@@ -404,6 +405,37 @@ struct Group {
         int nv = std::min(fq::resident_variants(pot, inter, ext, vp, vm, 64), 64);
         if (nv <= 0) {
             throw Error("no resident kernel for this potential / interaction / external combination");
+        }
+        // Chains without a requested shape: 4 particles per thread while the
+        // systems' wavefronts fit two per SIMD (a workgroup's step is shortest
+        // there: a lone 1024-chain's quasistatic step takes 2.37 us, against
+        // 2.63 us with 2 x 512 and 3.04 us with 1 x 1024), else 8 (the most
+        // particle-steps per unit of time with many systems: 1024 chains of
+        // 1024 particles), when a variant holds the largest system that way.
+        const bool chain = inter == FQSB_INT_NONE || inter == FQSB_INT_LAPLACE1D || inter == FQSB_INT_QUARTIC1D ||
+                           inter == FQSB_INT_QUARTICGRADIENT1D;
+        if (want_ppt == 0 && want_threads == 0 && chain) {
+            for (const int p : {4, 8}) {
+                const int thr = (int)((((N + p - 1) / p) + 63) / 64 * 64);
+                if (thr > 1024 || fq::resident_lds(inter, N, thr) > lds_limit) {
+                    continue;
+                }
+                if (p == 4 && R * (int64_t)(thr / 64) > 2 * 4 * (int64_t)cus) {
+                    continue;
+                }
+                int best = -1;
+                for (int i = 0; i < nv; ++i) {
+                    if (vp[i] == p && thr <= vm[i] && (best < 0 || vm[i] < vm[best])) {
+                        best = i;
+                    }
+                }
+                if (best >= 0) {
+                    ppt = p;
+                    maxt = vm[best];
+                    threads = thr;
+                    return;
+                }
+            }
         }
         int best = -1, best_thr = 0;
         for (int i = 0; i < nv; ++i) {
@@ -1764,6 +1796,7 @@ Group* build_group_multi(const std::vector<fqsb_desc>& ds, int64_t resident_max 
     hipDeviceProp_t prop;
     HIPC(hipGetDeviceProperties(&prop, dev));
     g->lds_limit = prop.sharedMemPerBlock - 2048;
+    g->cus = prop.multiProcessorCount;
     if (const char* gk = std::getenv("FQSB_GENERAL_KERNEL")) {
         g->general_kernel = gk[0] == '1';
     }

@@ -224,7 +224,10 @@ int fqsb_batch_create(const fqsb_desc* desc, uint64_t replicas, uint64_t seed_st
 int fqsb_batch_destroy(fqsb_batch* b);
 uint64_t fqsb_batch_replicas(const fqsb_batch* b);
 /* Choose the resident launch shape (particles per thread, threads per
- * workgroup); 0, 0 restores the automatic choice. */
+ * workgroup); 0, 0 restores the automatic choice (chains: 4 particles per
+ * thread while the replicas' wavefronts fit two per SIMD, else 8, where a
+ * kernel variant holds a replica so; otherwise the fewest particles per
+ * thread that hold it). Results do not depend on the shape. */
 int fqsb_batch_set_launch(fqsb_batch* b, int ppt, int threads);
 int fqsb_batch_get_launch(const fqsb_batch* b, int* ppt, int* threads);
 /* How a batch evaluates LongRange1d (detail.h:846-867):
