@@ -406,15 +406,18 @@ struct Group {
         if (nv <= 0) {
             throw Error("no resident kernel for this potential / interaction / external combination");
         }
-        // Chains without a requested shape: 4 particles per thread while the
-        // systems' wavefronts fit two per SIMD (a workgroup's step is shortest
-        // there: a lone 1024-chain's quasistatic step takes 2.37 us, against
-        // 2.63 us with 2 x 512 and 3.04 us with 1 x 1024), else 8 (the most
-        // particle-steps per unit of time with many systems: 1024 chains of
-        // 1024 particles), when a variant holds the largest system that way.
-        const bool chain = inter == FQSB_INT_NONE || inter == FQSB_INT_LAPLACE1D || inter == FQSB_INT_QUARTIC1D ||
-                           inter == FQSB_INT_QUARTICGRADIENT1D;
-        if (want_ppt == 0 && want_threads == 0 && chain) {
+        // Cuspy-Laplace chains (no external forcing) without a requested
+        // shape: 4 particles per thread while the systems' wavefronts fit two
+        // per SIMD (a workgroup's step is shortest there: a lone 1024-chain's
+        // quasistatic step takes 2.37 us, against 2.63 us with 2 x 512 and
+        // 3.04 us with 1 x 1024), else 8 (the most particle-steps per unit of
+        // time with many systems: 1024 chains of 1024 particles), when a
+        // variant of at most 512 threads holds the largest system that way
+        // (variants of 1024 threads have 128 registers per thread and spill).
+        // The other classes keep the fewest particles per thread: their
+        // costlier forces spill at 4 particles per thread.
+        if (want_ppt == 0 && want_threads == 0 && pot == FQSB_POT_CUSPY && inter == FQSB_INT_LAPLACE1D &&
+            ext == FQSB_EXT_NONE) {
             for (const int p : {4, 8}) {
                 const int thr = (int)((((N + p - 1) / p) + 63) / 64 * 64);
                 if (thr > 1024 || fq::resident_lds(inter, N, thr) > lds_limit) {
@@ -425,7 +428,7 @@ struct Group {
                 }
                 int best = -1;
                 for (int i = 0; i < nv; ++i) {
-                    if (vp[i] == p && thr <= vm[i] && (best < 0 || vm[i] < vm[best])) {
+                    if (vp[i] == p && thr <= vm[i] && vm[i] <= 512 && (best < 0 || vm[i] < vm[best])) {
                         best = i;
                     }
                 }
