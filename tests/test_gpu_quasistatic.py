@@ -36,6 +36,8 @@ def assert_same_events(q, u_frame, f_frame, S, inc, what=""):
 
 # (submodule, class, size, extra constructor arguments)
 SYSTEMS = [
+    ("Line1d", "System_Cuspy_Laplace", 3, dict(k_interactions=1.0)),
+    ("Line1d", "System_Cuspy_Laplace", 67, dict(k_interactions=1.0)),
     ("Line1d", "System_Cuspy_Laplace", 1000, dict(k_interactions=1.0)),
     ("Line1d", "System_Cuspy_Laplace", 2000, dict(k_interactions=1.0)),
     ("Line1d", "System_Cuspy_Laplace", 3001, dict(k_interactions=1.0)),
