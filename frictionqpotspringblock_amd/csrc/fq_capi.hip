@@ -1573,7 +1573,7 @@ struct Group {
         if (const char* e = std::getenv("FQSB_QS_BLOCK")) {
             ra.max_steps = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), (int64_t)MIN_BLOCK));
         }
-        const size_t lds = fq::resident_lds(inter, N, threads);
+        const size_t lds = fq::quasi_lds(inter, N, ppt, maxt, threads);
         bool unit_mass = true, full = true;
         for (const auto& S : hviews) {
             unit_mass &= S.inv_m == 1.0;

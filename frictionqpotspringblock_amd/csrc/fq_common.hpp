@@ -61,6 +61,15 @@ enum Status : int32_t {
     ST_MAXITER = 9    // quasistatic driver: a minimisation used max_iter steps
 };
 
+// The quasistatic driver for chains (k_quasi_chain) keeps the state of the
+// last committed step in LDS (8 bytes per particle, beyond the exchange
+// buffers) instead of a second register array for these launch variants
+// (particles per thread, max threads).
+FQ_HD constexpr bool quasi_inplace(int ppt, int maxt)
+{
+    return ppt == 8 && maxt <= 512;
+}
+
 // Phase of the quasistatic driver (k_quasi) within an event.
 enum QsPhase : int32_t {
     QS_BEGIN = 0,    // the event's displacement is still to be applied

@@ -7,7 +7,9 @@
 //  * software-pipelined: the slips of step it+1 are computed once at the end
 //    of step it (into the second register array), checked for reachability
 //    and published for the neighbours; step it+1 commits them. The loop is
-//    unrolled by two so that the arrays swap roles without copies. One
+//    unrolled by two so that the arrays swap roles without copies (with 8
+//    particles per thread one array: the committed state goes to LDS, read
+//    back only when the entry stops, and the next slips replace it). One
 //    __syncthreads per step, at its top, after which the StopList decision on
 //    step it-1 is taken from the wavefronts' residual partial sums;
 //  * the Cuspy mid-point of the current well kept in a register;
@@ -101,8 +103,14 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 
     // sa / sb: the state's slips and the slips prepared for the next step, in
     // turn; registers i >= cnt are padding
-    double sa[PPT], sb[PPT], v[PPT], a[PPT], ym[PPT];
+    // INPLACE (8 particles per thread, whose registers are the scarcest; up
+    // to 512 threads, for the LDS): one slip array; the state of the last
+    // committed step is kept in this thread's slots of LDS (read back only
+    // when the entry stops) and the next slips are prepared in place
+    constexpr bool INPLACE = quasi_inplace(PPT, MAXT);
+    double sa[PPT], sb[INPLACE ? 1 : PPT], v[PPT], a[PPT], ym[PPT];
     int32_t jl[PPT];
+    double* const s_keep = s_dyn + 4 * T; // INPLACE: [PPT][T] the state of the last committed step
     // the bounds (y_l, y_r) of each register's well and, for up to 4 particles
     // per thread (NEXT; 8 would spill), the right bound of the next well to
     // the right (yield position j + 2; +inf past the window's end), read when
@@ -143,6 +151,9 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         int out = 0;
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
+            if constexpr (INPLACE) {
+                s_keep[i * T + t] = x[i];
+            }
             y[i] = x[i] + dt * v[i] + c2 * a[i];
             double lo, hi;
             bounds(i, lo, hi);
@@ -590,7 +601,10 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         const SysView& W = V();
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            sa[i] = sb[i] = v[i] = a[i] = ym[i] = 0.0;
+            sa[i] = v[i] = a[i] = ym[i] = 0.0;
+            if constexpr (!INPLACE) {
+                sb[i] = 0.0;
+            }
             set_bounds(i, -__builtin_inf(), __builtin_inf());
             if constexpr (NEXT) {
                 yn_r[i] = __builtin_inf();
@@ -623,18 +637,32 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
         const bool minimising = !complete && phase == QS_MINIMISE;
         maxit = s_qa.max_iter - ev_done0;
         if (minimising) {
-            prepare(1, sa, sb);
-            for (;; it += 2) { // steps in pairs, the register arrays in turn; the state ends in sa
-                if (!step(it, sb, sa)) {
-                    break;
-                }
-                if (!step(it + 1, sa, sb)) {
-                    ++it;
-#pragma unroll
-                    for (int i = 0; i < PPT; ++i) {
-                        sa[i] = sb[i];
+            if constexpr (INPLACE) {
+                prepare(1, sa, sa);
+                for (;; ++it) {
+                    if (!step(it, sa, sa)) {
+                        break;
                     }
-                    break;
+                }
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    sa[i] = s_keep[i * T + t];
+                }
+            }
+            else {
+                prepare(1, sa, sb);
+                for (;; it += 2) { // steps in pairs, the register arrays in turn; the state ends in sa
+                    if (!step(it, sb, sa)) {
+                        break;
+                    }
+                    if (!step(it + 1, sa, sb)) {
+                        ++it;
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) {
+                            sa[i] = sb[i];
+                        }
+                        break;
+                    }
                 }
             }
         }
