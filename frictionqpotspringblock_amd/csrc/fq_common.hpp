@@ -67,7 +67,7 @@ enum Status : int32_t {
 // (particles per thread, max threads).
 FQ_HD constexpr bool quasi_inplace(int ppt, int maxt)
 {
-    return ppt == 8 && maxt <= 512;
+    return (ppt == 8 || ppt == 4) && maxt <= 512;
 }
 
 // Phase of the quasistatic driver (k_quasi) within an event.

@@ -7,8 +7,8 @@
 //  * software-pipelined: the slips of step it+1 are computed once at the end
 //    of step it (into the second register array), checked for reachability
 //    and published for the neighbours; step it+1 commits them. The loop is
-//    unrolled by two so that the arrays swap roles without copies (with 8
-//    particles per thread one array: the committed state goes to LDS, read
+//    unrolled by two so that the arrays swap roles without copies (with 4 or
+//    8 particles per thread one array: the committed state goes to LDS, read
 //    back only when the entry stops, and the next slips replace it). One
 //    __syncthreads per step, at its top, after which the StopList decision on
 //    step it-1 is taken from the wavefronts' residual partial sums;
@@ -103,10 +103,11 @@ k_quasi_chain(const SysView* __restrict__ views, RunArgs ra, QsArgs qa)
 
     // sa / sb: the state's slips and the slips prepared for the next step, in
     // turn; registers i >= cnt are padding
-    // INPLACE (8 particles per thread, whose registers are the scarcest; up
-    // to 512 threads, for the LDS): one slip array; the state of the last
-    // committed step is kept in this thread's slots of LDS (read back only
-    // when the entry stops) and the next slips are prepared in place
+    // INPLACE (4 or 8 particles per thread, up to 512 threads, for the LDS):
+    // one slip array; the state of the last committed step is kept in this
+    // thread's slots of LDS (read back only when the entry stops) and the
+    // next slips are prepared in place: no second register array (8
+    // particles per thread: no spills) and one step per loop iteration
     constexpr bool INPLACE = quasi_inplace(PPT, MAXT);
     double sa[PPT], sb[INPLACE ? 1 : PPT], v[PPT], a[PPT], ym[PPT];
     int32_t jl[PPT];
